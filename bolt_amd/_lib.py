@@ -1,0 +1,98 @@
+"""ctypes binding of ``libboltx_hip.so`` (declarations in include/boltx.h).
+
+Loading fails loudly: there is no fallback path.  Every call passes device
+pointers from torch tensors and the current torch stream.
+"""
+import ctypes
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libboltx_hip.so")
+
+RECON = {"piecewise-constant": 0, "minmod": 1, "ppm": 2, "weno5": 3}
+RIEMANN = {"upwind-flux": 0, "lax-friedrichs": 1}
+PRECISION = {"fp64": 0, "fp32-bulk": 1}
+
+EXPORTS = ("boltx_fvm_stage", "boltx_moments", "boltx_currents", "boltx_fdtd_step",
+           "boltx_yee_to_cell_centered", "boltx_check_finite", "boltx_version",
+           "boltx_last_error")
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("n_q1", ctypes.c_int32), ("n_q2", ctypes.c_int32),
+                ("n_p1", ctypes.c_int32), ("n_p2", ctypes.c_int32), ("n_p3", ctypes.c_int32),
+                ("n_species", ctypes.c_int32),
+                ("dq1", ctypes.c_double), ("dq2", ctypes.c_double),
+                ("dp1", ctypes.c_double), ("dp2", ctypes.c_double), ("dp3", ctypes.c_double),
+                ("p1_start", ctypes.c_double), ("p2_start", ctypes.c_double),
+                ("p3_start", ctypes.c_double)]
+
+
+class Physics(ctypes.Structure):
+    _fields_ = [("recon_q", ctypes.c_int32), ("recon_p", ctypes.c_int32),
+                ("riemann_q", ctypes.c_int32), ("riemann_p", ctypes.c_int32),
+                ("source_enabled", ctypes.c_int32), ("p_advection", ctypes.c_int32),
+                ("p_dim", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("mass", ctypes.c_double), ("charge", ctypes.c_double),
+                ("boltzmann_constant", ctypes.c_double), ("tau", ctypes.c_double)]
+
+
+class Stage(ctypes.Structure):
+    _fields_ = [("u", ctypes.c_void_p), ("w", ctypes.c_void_p), ("un", ctypes.c_void_p),
+                ("out", ctypes.c_void_p),
+                ("a", ctypes.c_double), ("b", ctypes.c_double), ("cdt", ctypes.c_double),
+                ("mom_in", ctypes.c_void_p), ("mom_out", ctypes.c_void_p),
+                ("fields_c_p1", ctypes.c_void_p), ("fields_c_p2", ctypes.c_void_p),
+                ("fields_c_p3", ctypes.c_void_p)]
+
+
+class BoltxError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load the HIP library (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise BoltxError(f"{path} is missing: build it with `python -m bolt_amd.build` "
+                         "(there is no fallback path)")
+    lib = ctypes.CDLL(path)
+    for name in EXPORTS:
+        getattr(lib, name)
+    vp, i32, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_double
+    g, ph, st = ctypes.POINTER(Grid), ctypes.POINTER(Physics), ctypes.POINTER(Stage)
+    lib.boltx_fvm_stage.argtypes = [g, ph, st, vp]
+    lib.boltx_moments.argtypes = [g, i32, vp, vp, vp]
+    lib.boltx_currents.argtypes = [g, ph, vp, vp, vp]
+    lib.boltx_fdtd_step.argtypes = [g, vp, vp, dbl, dbl, dbl, vp]
+    lib.boltx_yee_to_cell_centered.argtypes = [g, vp, vp, vp]
+    lib.boltx_check_finite.argtypes = [g, vp, vp, vp]
+    for name in EXPORTS[:-2]:
+        getattr(lib, name).restype = ctypes.c_int
+    lib.boltx_version.argtypes = []
+    lib.boltx_version.restype = ctypes.c_int
+    lib.boltx_last_error.argtypes = []
+    lib.boltx_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = _lib.boltx_last_error().decode() if _lib is not None else ""
+        raise BoltxError(f"boltx call failed ({rc}): {msg}")
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL)."""
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def stream_handle():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)

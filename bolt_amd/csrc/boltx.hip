@@ -1,0 +1,480 @@
+// gfx950 kernels and C-ABI of the nonlinear FVM step (see include/boltx.h).
+//
+// Work decomposition: one workgroup per q-cell (all P of it), so the velocity
+// moments of a stage's output are a deterministic in-workgroup reduction and
+// the p-direction stencils (fields on) read the q-cell's block from LDS.  The
+// q-direction stencil reads neighbouring q-cells' p-rows straight from HBM/L2;
+// workgroup ids are remapped so each XCD sweeps one contiguous band of q2 rows
+// and its L2 holds the rows the stencil revisits.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "../../include/boltx.h"
+#include "fvm_stage.h"
+
+namespace bx {
+
+static thread_local char g_err[512];
+
+static int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+static int fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+
+// blockIdx -> q-cell: consecutive workgroups are dispatched round-robin over
+// the 8 XCDs; give XCD x the contiguous range [x*Q/8, (x+1)*Q/8).
+__device__ inline int xcd_map(int b, int nb) {
+  if (nb % 8 != 0) return b;
+  return (b % 8) * (nb / 8) + b / 8;
+}
+
+// Sum of ``v`` over the workgroup, deterministic (butterfly within waves,
+// then a fixed-order sum over waves).  Result valid in thread 0.
+template <int N>
+__device__ inline void block_sum(double (&v)[N], double (*red)[kWaves]) {
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    double x = v[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    v[k] = x;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) red[k][wave] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      double s = red[k][0];
+      for (int w = 1; w < kWaves; ++w) s += red[k][w];
+      v[k] = s;
+    }
+  }
+}
+
+// Maxwellian factor tables of q-cell q into LDS (collision_operator.py:52-65).
+template <typename T>
+__device__ inline void build_tables(const StageParams& sp, int q, T* tab, QCell<T>& c) {
+  const BgkCell b = qcell_bgk(sp, q);
+  c.pref = maxwell_prefactor(b, sp.mass, sp.kB, sp.p_dim);
+  T* e1 = tab;
+  T* e2 = tab + sp.n_p1;
+  T* e3 = e2 + sp.n_p2;
+  for (int i = threadIdx.x; i < sp.n_p1; i += blockDim.x)
+    e1[i] = T(maxwell_factor(centre(sp.p1s, i, sp.dp1), b.u1, b.T, sp.mass, sp.kB));
+  for (int i = threadIdx.x; i < sp.n_p2; i += blockDim.x)
+    e2[i] = T(maxwell_factor(centre(sp.p2s, i, sp.dp2), b.u2, b.T, sp.mass, sp.kB));
+  for (int i = threadIdx.x; i < sp.n_p3; i += blockDim.x)
+    e3[i] = T(maxwell_factor(centre(sp.p3s, i, sp.dp3), b.u3, b.T, sp.mass, sp.kB));
+  c.e1 = e1;
+  c.e2 = e2;
+  c.e3 = e3;
+}
+
+// Accumulate the five built-in moments of value x at velocity v
+// (moments.py:5-22): f, f v1, f v2, f v3, 0.5 f (v1^2 + v2^2 + v3^2).
+__device__ inline void add_moments(double (&acc)[5], double x, const double v[3]) {
+  acc[0] += x;
+  acc[1] += x * v[0];
+  acc[2] += x * v[1];
+  acc[3] += x * v[2];
+  acc[4] += (0.5 * x) * ((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+}
+
+template <typename T, int RQ, int RP, bool PADV>
+__global__ __launch_bounds__(kBlock) void fvm_stage_kernel(StageParams sp) {
+  extern __shared__ unsigned char smem[];
+  __shared__ double red[5][kWaves];
+  T* tab = reinterpret_cast<T*>(smem);
+  T* blk = tab + (sp.n_p1 + sp.n_p2 + sp.n_p3);
+  const int q = xcd_map(blockIdx.x, gridDim.x);
+  QCell<T> c;
+  qcell_neighbours(sp, q, c);
+  c.pref = 0.0;
+  c.e1 = c.e2 = c.e3 = tab;
+  if (sp.source) build_tables(sp, q, tab, c);
+  if constexpr (PADV) {
+    qcell_fields(sp, c);
+    const T* u = (const T*)sp.u + (long)q * sp.P;
+    for (int p = threadIdx.x; p < sp.P; p += blockDim.x) blk[p] = u[p];
+  }
+  __syncthreads();
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  T* out = (T*)sp.out + (long)q * sp.P;
+  for (int p = threadIdx.x; p < sp.P; p += blockDim.x) {
+    double v[3];
+    const T x = stage_cell<T, RQ, RP, PADV>(sp, c, p, blk, v);
+    out[p] = x;
+    add_moments(acc, double(x), v);
+  }
+  if (sp.mom_out) {
+    block_sum(acc, red);
+    if (threadIdx.x == 0) {
+      for (int k = 0; k < 5; ++k) sp.mom_out[k * sp.Q + q] = acc[k] * sp.measure;
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void moments_kernel(StageParams sp) {
+  __shared__ double red[5][kWaves];
+  const int q = xcd_map(blockIdx.x, gridDim.x);
+  const T* f = (const T*)sp.u + (long)q * sp.P;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int p = threadIdx.x; p < sp.P; p += blockDim.x) {
+    const int j1 = p % sp.n_p1, j23 = p / sp.n_p1;
+    const double v[3] = {centre(sp.p1s, j1, sp.dp1), centre(sp.p2s, j23 % sp.n_p2, sp.dp2),
+                         centre(sp.p3s, j23 / sp.n_p2, sp.dp3)};
+    add_moments(acc, double(f[p]), v);
+  }
+  block_sum(acc, red);
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < 5; ++k) sp.mom_out[k * sp.Q + q] = acc[k] * sp.measure;
+  }
+}
+
+// J1 = e * sum(f_left v1) dV, J2 = e * sum(f_bot v2) dV, J3 = e * sum(f v3) dV
+// with f_left / f_bot the upwind states of the q1 / q2 left faces
+// (df_dt_fvm.py:79-80, 214-224).
+template <typename T, int RQ>
+__global__ __launch_bounds__(kBlock) void currents_kernel(StageParams sp, double* J) {
+  __shared__ double red[3][kWaves];
+  const int q = xcd_map(blockIdx.x, gridDim.x);
+  QCell<T> c;
+  qcell_neighbours(sp, q, c);
+  const T* u = (const T*)sp.u;
+  const long P = sp.P;
+  constexpr int r = recon_reach(RQ) + 1;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int p = threadIdx.x; p < sp.P; p += blockDim.x) {
+    const int j1 = p % sp.n_p1, j23 = p / sp.n_p1;
+    const double v1 = centre(sp.p1s, j1, sp.dp1);
+    const double v2 = centre(sp.p2s, j23 % sp.n_p2, sp.dp2);
+    const double v3 = centre(sp.p3s, j23 / sp.n_p2, sp.dp3);
+    const T f = u[(long)q * P + p];
+    T s1 = f, s2 = f;
+    if (sp.n_q1 > 1) {
+      T a[7];
+      for (int k = 3 - r; k <= 3 + r; ++k) a[k] = k == 3 ? f : u[(long)c.nb1[k] * P + p];
+      flux_diff<RQ, T>(a, T(v1), RS_UPWIND, &s1);
+    }
+    if (sp.n_q2 > 1) {
+      T a[7];
+      for (int k = 3 - r; k <= 3 + r; ++k) a[k] = k == 3 ? f : u[(long)c.nb2[k] * P + p];
+      flux_diff<RQ, T>(a, T(v2), RS_UPWIND, &s2);
+    }
+    acc[0] += double(s1) * v1;
+    acc[1] += double(s2) * v2;
+    acc[2] += double(f) * v3;
+  }
+  block_sum(acc, red);
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < 3; ++k) J[k * sp.Q + q] = sp.charge * (acc[k] * sp.measure);
+  }
+}
+
+// FDTD on the periodic Yee grid (fdtd_explicit.py:9-138).  Component k of
+// yee / cc / J is the [Q] slice k*Q.., index i1 + n_q1*i2.
+struct Grid2 {
+  int n1, n2;
+  __device__ int at(int i1, int i2) const {
+    i1 = (i1 + n1) % n1;
+    i2 = (i2 + n2) % n2;
+    return i1 + n1 * i2;
+  }
+};
+
+__global__ void fdtd_e_kernel(Grid2 g, double* y, const double* J, double dq1, double dq2,
+                              double dt, double eps, double mu) {
+  const int Q = g.n1 * g.n2;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= Q) return;
+  const int i1 = t % g.n1, i2 = t / g.n1;
+  const double* B1 = y + 3 * Q;
+  const double* B2 = y + 4 * Q;
+  const double* B3 = y + 5 * Q;
+  const double curlB_1 = (B3[g.at(i1, i2 + 1)] - B3[t]) / dq2;
+  const double curlB_2 = -(B3[g.at(i1 + 1, i2)] - B3[t]) / dq1;
+  const double curlB_3 = (B2[g.at(i1 + 1, i2)] - B2[t]) / dq1 - (B1[g.at(i1, i2 + 1)] - B1[t]) / dq2;
+  const double c = dt / (mu * eps);
+  y[t] = y[t] + (c * curlB_1 - J[t] * dt / eps);
+  y[Q + t] = y[Q + t] + (c * curlB_2 - J[Q + t] * dt / eps);
+  y[2 * Q + t] = y[2 * Q + t] + (c * curlB_3 - J[2 * Q + t] * dt / eps);
+}
+
+__global__ void fdtd_b_kernel(Grid2 g, double* y, double dq1, double dq2, double dt) {
+  const int Q = g.n1 * g.n2;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= Q) return;
+  const int i1 = t % g.n1, i2 = t / g.n1;
+  const double* E1 = y;
+  const double* E2 = y + Q;
+  const double* E3 = y + 2 * Q;
+  const double curlE_1 = (E3[t] - E3[g.at(i1, i2 - 1)]) / dq2;
+  const double curlE_2 = -(E3[t] - E3[g.at(i1 - 1, i2)]) / dq1;
+  const double curlE_3 = (E2[t] - E2[g.at(i1 - 1, i2)]) / dq1 - (E1[t] - E1[g.at(i1, i2 - 1)]) / dq2;
+  y[3 * Q + t] = y[3 * Q + t] + -dt * curlE_1;
+  y[4 * Q + t] = y[4 * Q + t] + -dt * curlE_2;
+  y[5 * Q + t] = y[5 * Q + t] + -dt * curlE_3;
+}
+
+// fields.py:527-557.
+__global__ void yee_to_cc_kernel(Grid2 g, const double* y, double* cc) {
+  const int Q = g.n1 * g.n2;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= Q) return;
+  const int i1 = t % g.n1, i2 = t / g.n1;
+  const int p1 = g.at(i1 + 1, i2), p2 = g.at(i1, i2 + 1), p12 = g.at(i1 + 1, i2 + 1);
+  cc[t] = 0.5 * (y[t] + y[p1]);
+  cc[Q + t] = 0.5 * (y[Q + t] + y[Q + p2]);
+  cc[2 * Q + t] = y[2 * Q + t];
+  cc[3 * Q + t] = 0.5 * (y[3 * Q + t] + y[3 * Q + p2]);
+  cc[4 * Q + t] = 0.5 * (y[4 * Q + t] + y[4 * Q + p1]);
+  cc[5 * Q + t] = 0.25 * (y[5 * Q + t] + y[5 * Q + p2] + y[5 * Q + p1] + y[5 * Q + p12]);
+}
+
+__global__ void check_finite_kernel(const double* n, int Q, int* flag) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < Q && !isfinite(n[t])) flag[0] = 1;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+static int check_grid(const boltx_grid* g) {
+  if (!g) return fail(BOLTX_EINVAL, "grid is NULL");
+  if (g->n_q1 < 1 || g->n_q2 < 1 || g->n_p1 < 1 || g->n_p2 < 1 || g->n_p3 < 1)
+    return fail(BOLTX_EINVAL, "grid sizes must be positive (got q %d x %d, p %d x %d x %d)",
+                g->n_q1, g->n_q2, g->n_p1, g->n_p2, g->n_p3);
+  if (g->n_species != 1)
+    return fail(BOLTX_EUNSUPPORTED, "n_species = %d: only one species is supported", g->n_species);
+  const long P = (long)g->n_p1 * g->n_p2 * g->n_p3;
+  const long Q = (long)g->n_q1 * g->n_q2;
+  if (P * Q > (1L << 40)) return fail(BOLTX_EINVAL, "grid too large");
+  if (Q > 0x7fffffff) return fail(BOLTX_EINVAL, "too many q-cells");
+  return BOLTX_OK;
+}
+
+static StageParams base_params(const boltx_grid* g) {
+  StageParams sp;
+  memset(&sp, 0, sizeof(sp));
+  sp.n_q1 = g->n_q1;
+  sp.n_q2 = g->n_q2;
+  sp.n_p1 = g->n_p1;
+  sp.n_p2 = g->n_p2;
+  sp.n_p3 = g->n_p3;
+  sp.P = g->n_p1 * g->n_p2 * g->n_p3;
+  sp.Q = g->n_q1 * g->n_q2;
+  sp.p1s = g->p1_start;
+  sp.p2s = g->p2_start;
+  sp.p3s = g->p3_start;
+  sp.dp1 = g->dp1;
+  sp.dp2 = g->dp2;
+  sp.dp3 = g->dp3;
+  sp.inv_dq1 = 1.0 / g->dq1;
+  sp.inv_dq2 = 1.0 / g->dq2;
+  sp.inv_dp1 = 1.0 / g->dp1;
+  sp.inv_dp2 = 1.0 / g->dp2;
+  sp.inv_dp3 = 1.0 / g->dp3;
+  sp.measure = g->dp3 * g->dp2 * g->dp1;
+  return sp;
+}
+
+static int launch_status(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(BOLTX_ELAUNCH, "%s: %s", what, hipGetErrorString(e));
+  return BOLTX_OK;
+}
+
+template <typename T, int RQ, int RP, bool PADV>
+static int launch_stage_t(const StageParams& sp, hipStream_t s) {
+  size_t lds = sizeof(T) * (size_t)(sp.n_p1 + sp.n_p2 + sp.n_p3);
+  if (PADV) lds += sizeof(T) * (size_t)sp.P;
+  if (lds > 160 * 1024) return fail(BOLTX_EUNSUPPORTED, "p-block of %zu bytes exceeds LDS", lds);
+  hipLaunchKernelGGL((fvm_stage_kernel<T, RQ, RP, PADV>), dim3(sp.Q), dim3(kBlock), lds, s, sp);
+  return launch_status("fvm_stage_kernel");
+}
+
+template <typename T, int RQ>
+static int launch_stage_rp(const StageParams& sp, int rp, bool padv, hipStream_t s) {
+  if (!padv) return launch_stage_t<T, RQ, RC_PC, false>(sp, s);
+  switch (rp) {
+    case RC_PC: return launch_stage_t<T, RQ, RC_PC, true>(sp, s);
+    case RC_MINMOD: return launch_stage_t<T, RQ, RC_MINMOD, true>(sp, s);
+    case RC_PPM: return launch_stage_t<T, RQ, RC_PPM, true>(sp, s);
+    case RC_WENO5: return launch_stage_t<T, RQ, RC_WENO5, true>(sp, s);
+  }
+  return fail(BOLTX_EINVAL, "recon_p = %d", rp);
+}
+
+template <typename T>
+static int launch_stage_rq(const StageParams& sp, int rq, int rp, bool padv, hipStream_t s) {
+  switch (rq) {
+    case RC_PC: return launch_stage_rp<T, RC_PC>(sp, rp, padv, s);
+    case RC_MINMOD: return launch_stage_rp<T, RC_MINMOD>(sp, rp, padv, s);
+    case RC_PPM: return launch_stage_rp<T, RC_PPM>(sp, rp, padv, s);
+    case RC_WENO5: return launch_stage_rp<T, RC_WENO5>(sp, rp, padv, s);
+  }
+  return fail(BOLTX_EINVAL, "recon_q = %d", rq);
+}
+
+static int check_physics(const boltx_physics* ph) {
+  if (!ph) return fail(BOLTX_EINVAL, "physics is NULL");
+  if (ph->recon_q < 0 || ph->recon_q > 3 || ph->recon_p < 0 || ph->recon_p > 3)
+    return fail(BOLTX_EINVAL, "reconstruction out of range (q %d, p %d)", ph->recon_q, ph->recon_p);
+  if (ph->riemann_q < 0 || ph->riemann_q > 1 || ph->riemann_p < 0 || ph->riemann_p > 1)
+    return fail(BOLTX_EINVAL, "riemann solver out of range");
+  if (ph->p_dim < 1 || ph->p_dim > 3) return fail(BOLTX_EINVAL, "p_dim = %d", ph->p_dim);
+  if (ph->precision != BOLTX_FP64 && ph->precision != BOLTX_FP32_BULK)
+    return fail(BOLTX_EINVAL, "precision = %d", ph->precision);
+  if (ph->source_enabled && !(ph->tau > 0.0)) return fail(BOLTX_EINVAL, "tau must be > 0");
+  if (ph->mass == 0.0) return fail(BOLTX_EINVAL, "mass must be non-zero");
+  return BOLTX_OK;
+}
+
+}  // namespace bx
+
+using namespace bx;
+
+extern "C" int boltx_version(void) { return BOLTX_VERSION; }
+
+extern "C" const char* boltx_last_error(void) { return g_err; }
+
+extern "C" int boltx_fvm_stage(const boltx_grid* grid, const boltx_physics* phys,
+                               const boltx_stage* st, void* stream) {
+  int rc = check_grid(grid);
+  if (rc) return rc;
+  if ((rc = check_physics(phys))) return rc;
+  if (!st || !st->u || !st->w || !st->out) return fail(BOLTX_EINVAL, "stage pointers missing");
+  if (st->out == st->u) return fail(BOLTX_EINVAL, "out must not alias the stencil input u");
+  if (st->a != 0.0 && !st->un) return fail(BOLTX_EINVAL, "un is NULL but a != 0");
+  if (phys->source_enabled && !st->mom_in) return fail(BOLTX_EINVAL, "mom_in is NULL");
+  if (phys->p_advection && (!st->fields_c_p1 || !st->fields_c_p2 || !st->fields_c_p3))
+    return fail(BOLTX_EINVAL, "fields are NULL with p_advection on");
+  const int need = recon_reach(phys->recon_q) + 1;
+  if ((grid->n_q1 > 1 && grid->n_q1 < need) || (grid->n_q2 > 1 && grid->n_q2 < need))
+    return fail(BOLTX_EINVAL, "q-grid smaller than the stencil reach");
+  StageParams sp = base_params(grid);
+  sp.riemann_q = phys->riemann_q;
+  sp.riemann_p = phys->riemann_p;
+  sp.source = phys->source_enabled;
+  sp.p_dim = phys->p_dim;
+  sp.mass = phys->mass;
+  sp.charge = phys->charge;
+  sp.kB = phys->boltzmann_constant;
+  sp.inv_tau = phys->source_enabled ? 1.0 / phys->tau : 0.0;
+  sp.a = st->a;
+  sp.b = st->b;
+  sp.cdt = st->cdt;
+  sp.u = st->u;
+  sp.w = st->w;
+  sp.un = st->un;
+  sp.out = st->out;
+  sp.mom_in = st->mom_in;
+  sp.mom_out = st->mom_out;
+  sp.fp1 = st->fields_c_p1;
+  sp.fp2 = st->fields_c_p2;
+  sp.fp3 = st->fields_c_p3;
+  hipStream_t s = (hipStream_t)stream;
+  const bool padv = phys->p_advection != 0;
+  if (phys->precision == BOLTX_FP64)
+    return launch_stage_rq<double>(sp, phys->recon_q, phys->recon_p, padv, s);
+  return launch_stage_rq<float>(sp, phys->recon_q, phys->recon_p, padv, s);
+}
+
+extern "C" int boltx_moments(const boltx_grid* grid, int32_t precision, const void* f,
+                             double* mom, void* stream) {
+  int rc = check_grid(grid);
+  if (rc) return rc;
+  if (!f || !mom) return fail(BOLTX_EINVAL, "moments pointers missing");
+  StageParams sp = base_params(grid);
+  sp.u = f;
+  sp.mom_out = mom;
+  hipStream_t s = (hipStream_t)stream;
+  if (precision == BOLTX_FP64)
+    hipLaunchKernelGGL(moments_kernel<double>, dim3(sp.Q), dim3(kBlock), 0, s, sp);
+  else if (precision == BOLTX_FP32_BULK)
+    hipLaunchKernelGGL(moments_kernel<float>, dim3(sp.Q), dim3(kBlock), 0, s, sp);
+  else
+    return fail(BOLTX_EINVAL, "precision = %d", precision);
+  return launch_status("moments_kernel");
+}
+
+template <typename T>
+static int launch_currents(const StageParams& sp, int rq, double* J, hipStream_t s) {
+  switch (rq) {
+    case RC_PC: hipLaunchKernelGGL((currents_kernel<T, RC_PC>), dim3(sp.Q), dim3(kBlock), 0, s, sp, J); break;
+    case RC_MINMOD: hipLaunchKernelGGL((currents_kernel<T, RC_MINMOD>), dim3(sp.Q), dim3(kBlock), 0, s, sp, J); break;
+    case RC_PPM: hipLaunchKernelGGL((currents_kernel<T, RC_PPM>), dim3(sp.Q), dim3(kBlock), 0, s, sp, J); break;
+    case RC_WENO5: hipLaunchKernelGGL((currents_kernel<T, RC_WENO5>), dim3(sp.Q), dim3(kBlock), 0, s, sp, J); break;
+    default: return fail(BOLTX_EINVAL, "recon_q = %d", rq);
+  }
+  return launch_status("currents_kernel");
+}
+
+extern "C" int boltx_currents(const boltx_grid* grid, const boltx_physics* phys, const void* u,
+                              double* J, void* stream) {
+  int rc = check_grid(grid);
+  if (rc) return rc;
+  if ((rc = check_physics(phys))) return rc;
+  if (!u || !J) return fail(BOLTX_EINVAL, "currents pointers missing");
+  StageParams sp = base_params(grid);
+  sp.u = u;
+  sp.charge = phys->charge;
+  hipStream_t s = (hipStream_t)stream;
+  if (phys->precision == BOLTX_FP64) return launch_currents<double>(sp, phys->recon_q, J, s);
+  return launch_currents<float>(sp, phys->recon_q, J, s);
+}
+
+extern "C" int boltx_fdtd_step(const boltx_grid* grid, double* yee, const double* J, double dt,
+                               double eps, double mu, void* stream) {
+  int rc = check_grid(grid);
+  if (rc) return rc;
+  if (!yee || !J) return fail(BOLTX_EINVAL, "fdtd pointers missing");
+  Grid2 g{grid->n_q1, grid->n_q2};
+  const int Q = g.n1 * g.n2;
+  hipStream_t s = (hipStream_t)stream;
+  const int nb = (Q + 255) / 256;
+  hipLaunchKernelGGL(fdtd_e_kernel, dim3(nb), dim3(256), 0, s, g, yee, J, grid->dq1, grid->dq2,
+                     dt, eps, mu);
+  if ((rc = launch_status("fdtd_e_kernel"))) return rc;
+  hipLaunchKernelGGL(fdtd_b_kernel, dim3(nb), dim3(256), 0, s, g, yee, grid->dq1, grid->dq2, dt);
+  return launch_status("fdtd_b_kernel");
+}
+
+extern "C" int boltx_yee_to_cell_centered(const boltx_grid* grid, const double* yee, double* cc,
+                                          void* stream) {
+  int rc = check_grid(grid);
+  if (rc) return rc;
+  if (!yee || !cc || yee == cc) return fail(BOLTX_EINVAL, "yee/cc pointers invalid");
+  Grid2 g{grid->n_q1, grid->n_q2};
+  const int Q = g.n1 * g.n2;
+  hipLaunchKernelGGL(yee_to_cc_kernel, dim3((Q + 255) / 256), dim3(256), 0, (hipStream_t)stream, g,
+                     yee, cc);
+  return launch_status("yee_to_cc_kernel");
+}
+
+extern "C" int boltx_check_finite(const boltx_grid* grid, const double* mom, int32_t* flag,
+                                  void* stream) {
+  int rc = check_grid(grid);
+  if (rc) return rc;
+  if (!mom || !flag) return fail(BOLTX_EINVAL, "check_finite pointers missing");
+  const int Q = grid->n_q1 * grid->n_q2;
+  hipLaunchKernelGGL(check_finite_kernel, dim3((Q + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                     mom, Q, flag);
+  return launch_status("check_finite_kernel");
+}

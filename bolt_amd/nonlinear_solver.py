@@ -1,0 +1,347 @@
+"""Drop-in mirror of bolt/lib/nonlinear/nonlinear_solver.py (FVM strategy),
+with the time step running in the gfx950 HIP library.
+
+User code keeps the reference interface: ``nonlinear_solver(system)``,
+``strang_timestep(dt)`` (and lie/swss/jia, identical for FVM in q and p,
+timestep.py:13-15), ``compute_moments(name, f=None)``, ``f``, ``time_elapsed``,
+the grids ``q1_center`` ... ``p3_center`` and ``fields_solver``.  Arrays are
+torch tensors on the GPU in the reference's axis order: f is
+(N_p, N_s, N_q1 + 2 N_g, N_q2 + 2 N_g), q-grids are (1, 1, Q1g, Q2g) and
+p-grids are (N_p, N_s, 1, 1).
+
+Internally f is stored without ghost zones as a contiguous [N_q2][N_q1][N_p]
+array (periodic q is handled by modular indexing in the kernels), FP64 or -- with
+``params.precision = 'fp32-bulk'`` -- FP32 with FP64 moments.  There is no CPU
+path: construction fails if the HIP library or the GPU is missing.
+
+Additions to the reference's parameters (all optional):
+  ``riemann_solver_in_q/p``: 'upwind-flux' (reference) or 'lax-friedrichs';
+  ``time_stepper_fvm``: 'RK2' (reference, fvm_operator.py:6-35) or 'SSP-RK3';
+  ``precision``: 'fp64' (default) or 'fp32-bulk'.
+"""
+import ctypes
+import math
+import types
+
+import torch
+
+from . import _lib
+from .fields import fields_solver as _fields_solver
+from .src.nonrelativistic_boltzmann import advection_terms as _adv
+from .src.nonrelativistic_boltzmann import collision_operator as _coll
+from .src.nonrelativistic_boltzmann import moments as _mom
+
+F64 = torch.float64
+
+
+class nonlinear_solver(object):
+    """nonlinear_solver.py:55-521 for the FVM path on one GPU."""
+
+    def __init__(self, physical_system, performance_test_flag=False):
+        self._L = _lib.load()
+        if not torch.cuda.is_available():
+            raise RuntimeError('bolt_amd runs on a HIP GPU; none is visible (no CPU path)')
+        self.device = torch.device('cuda', torch.cuda.current_device())
+        ps = self.physical_system = physical_system
+        up = ps.params
+        # nonlinear_solver.py:100-121
+        self.q1_start, self.q1_end = ps.q1_start, ps.q1_end
+        self.q2_start, self.q2_end = ps.q2_start, ps.q2_end
+        self.N_q1, self.dq1 = ps.N_q1, ps.dq1
+        self.N_q2, self.dq2 = ps.N_q2, ps.dq2
+        self.N_p1, self.N_p2, self.N_p3 = ps.N_p1, ps.N_p2, ps.N_p3
+        self.N_ghost = ps.N_ghost
+        self.boundary_conditions = bc = ps.boundary_conditions
+        self.N_species = ps.N_species
+        self.performance_test_flag = performance_test_flag
+        if self.N_species != 1:
+            raise NotImplementedError('the fused kernels support one species')
+        # nonlinear_solver.py:177-249: periodic / none boundaries only
+        for side in ('in_q1_left', 'in_q1_right', 'in_q2_bottom', 'in_q2_top'):
+            if getattr(bc, side) not in ('periodic', 'none'):
+                raise NotImplementedError('Unavailable/Invalid boundary condition')
+        for a, b in (('in_q1_left', 'in_q1_right'), ('in_q2_bottom', 'in_q2_top')):
+            if getattr(bc, a) != getattr(bc, b):
+                raise Exception('Periodic boundary conditions need to be applied to '
+                                'both the boundaries of a particular axis')
+        self.N_g1 = 0 if bc.in_q1_left == 'none' else self.N_ghost
+        self.N_g2 = 0 if bc.in_q2_bottom == 'none' else self.N_ghost
+        if bc.in_q1_left == 'none' and self.N_q1 != 1:
+            raise NotImplementedError("'none' boundaries are supported on singleton axes")
+        if bc.in_q2_bottom == 'none' and self.N_q2 != 1:
+            raise NotImplementedError("'none' boundaries are supported on singleton axes")
+
+        # params as the callbacks see them: mass / charge as (1, N_s, 1, 1)
+        # tensors (nonlinear_solver.py:133-145), the user's module untouched.
+        self.params = types.SimpleNamespace(
+            **{k: getattr(up, k) for k in dir(up) if not k.startswith('__')})
+        self._mass = float(up.mass[0])
+        self._charge = float(up.charge[0])
+        self.params.mass = torch.tensor([list(up.mass)], dtype=F64,
+                                        device=self.device).reshape(1, -1, 1, 1)
+        self.params.charge = torch.tensor([list(up.charge)], dtype=F64,
+                                          device=self.device).reshape(1, -1, 1, 1)
+        p = self.params
+
+        self._configure(ps, p)
+        self._build_grids(ps)
+        self._grid = _lib.Grid(self.N_q1, self.N_q2, self.N_p1, self.N_p2, self.N_p3, 1,
+                               float(self.dq1), float(self.dq2), self._dp[0], self._dp[1],
+                               self._dp[2], float(ps.p1_start[0]), float(ps.p2_start[0]),
+                               float(ps.p3_start[0]))
+        self._phys = _lib.Physics(self._recon_q, self._recon_p, self._riemann_q,
+                                  self._riemann_p, int(self._source_on), int(self._p_adv),
+                                  int(getattr(p, 'p_dim', 1)), self._precision,
+                                  self._mass, self._charge,
+                                  float(getattr(p, 'boltzmann_constant', 1.0)),
+                                  float(self._tau))
+        Q, P = self.N_q1 * self.N_q2, self.N_p1 * self.N_p2 * self.N_p3
+        self._Q, self._P = Q, P
+        self._bufs = {}
+        self._mom = torch.zeros(5, Q, dtype=F64, device=self.device)
+        self._flag = torch.zeros(1, dtype=torch.int32, device=self.device)
+        self.time_elapsed = 0
+        self.dt = None
+        self._A_q, self._C_q = ps.A_q, ps.C_q
+        self._A_p, self._C_p = ps.A_p, ps.C_p
+        self._source = ps.source
+        self._initialize(p)
+        self._timings = {'time_ts': 0.0, 'steps': 0}
+
+    # -- configuration ------------------------------------------------------
+    def _configure(self, ps, p):
+        if getattr(p, 'solver_method_in_q', 'FVM') != 'FVM':
+            raise NotImplementedError('only the FVM strategy is provided')
+        rq = p.reconstruction_method_in_q
+        rp = getattr(p, 'reconstruction_method_in_p', rq)
+        if rq not in _lib.RECON or rp not in _lib.RECON:
+            raise NotImplementedError('Reconstruction method invalid/not-implemented')
+        self._recon_q, self._recon_p = _lib.RECON[rq], _lib.RECON[rp]
+        rsq = getattr(p, 'riemann_solver_in_q', 'upwind-flux')
+        rsp = getattr(p, 'riemann_solver_in_p', 'upwind-flux')
+        if rsq not in _lib.RIEMANN or rsp not in _lib.RIEMANN:
+            raise NotImplementedError('Riemann solver invalid/not-implemented')
+        self._riemann_q, self._riemann_p = _lib.RIEMANN[rsq], _lib.RIEMANN[rsp]
+        prec = getattr(p, 'precision', 'fp64')
+        if prec not in _lib.PRECISION:
+            raise NotImplementedError('precision must be fp64 or fp32-bulk')
+        self._precision = _lib.PRECISION[prec]
+        self._dtype = torch.float64 if prec == 'fp64' else torch.float32
+        self._stepper = getattr(p, 'time_stepper_fvm', 'RK2')
+        if self._stepper not in ('RK2', 'SSP-RK3'):
+            raise NotImplementedError('time_stepper_fvm must be RK2 or SSP-RK3')
+        if ps.C_q is not _adv.C_q:
+            raise NotImplementedError('the fused kernel implements the built-in C_q '
+                                      '(bolt_amd.src.nonrelativistic_boltzmann)')
+        self._source_on = bool(getattr(p, 'source_enabled', False))
+        if self._source_on:
+            if ps.source is not _coll.BGK:
+                raise NotImplementedError('the fused kernel implements the built-in BGK source')
+            if ps.moments is not _mom:
+                raise NotImplementedError('BGK needs the built-in moments module')
+        self._fields_on = bool(getattr(p, 'fields_enabled', False))
+        self._p_adv = self._fields_on and getattr(p, 'solver_method_in_p', 'FVM') == 'FVM'
+        if self._fields_on:
+            if getattr(p, 'fields_type', None) != 'electrodynamic':
+                raise NotImplementedError('electrostatic fields are not provided')
+            if getattr(p, 'fields_solver', '').lower() != 'fdtd':
+                raise NotImplementedError('only the FDTD field solver is provided')
+            if getattr(p, 'hybrid_model_enabled', False):
+                raise NotImplementedError('the hybrid model is not provided')
+            if self._p_adv and ps.C_p is not _adv.C_p:
+                raise NotImplementedError('the fused kernel implements the built-in C_p')
+
+    def _build_grids(self, ps):
+        """calculate_q.py:7-41 and calculate_p.py:8-57 as device tensors."""
+        dev, S = self.device, 1
+        i1 = torch.arange(-self.N_g1, self.N_q1 + self.N_g1, dtype=F64, device=dev)
+        i2 = torch.arange(-self.N_g2, self.N_q2 + self.N_g2, dtype=F64, device=dev)
+        Q1g, Q2g = len(i1), len(i2)
+        q1lb = (self.q1_start + i1 * self.dq1).reshape(1, 1, Q1g, 1).expand(1, 1, Q1g, Q2g)
+        q2lb = (self.q2_start + i2 * self.dq2).reshape(1, 1, 1, Q2g).expand(1, 1, Q1g, Q2g)
+        self.q1_left_bot, self.q2_left_bot = q1lb.contiguous(), q2lb.contiguous()
+        self.q1_center_bot = self.q1_left_bot + 0.5 * self.dq1
+        self.q2_center_bot = self.q2_left_bot
+        self.q1_left_center = self.q1_left_bot
+        self.q2_left_center = self.q2_center_bot  # calculate_q.py:37
+        self.q1_center = self.q1_left_bot + 0.5 * self.dq1
+        self.q2_center = self.q2_left_bot + 0.5 * self.dq2
+        dp = [float(ps.dp1[0]), float(ps.dp2[0]), float(ps.dp3[0])]
+        starts = [float(ps.p1_start[0]), float(ps.p2_start[0]), float(ps.p3_start[0])]
+        N = [self.N_p1, self.N_p2, self.N_p3]
+        self._dp = dp
+
+        def mesh(offset):
+            ax = [starts[k] + (offset + torch.arange(N[k], dtype=F64, device=dev)) * dp[k]
+                  for k in range(3)]
+            g3, g2, g1 = torch.meshgrid(ax[2], ax[1], ax[0], indexing='ij')
+            return [g.reshape(-1, S, 1, 1).contiguous() for g in (g1, g2, g3)]
+
+        self.p1_center, self.p2_center, self.p3_center = mesh(0.5)
+        self.p1_left, self.p2_bottom, self.p3_back = mesh(0.0)
+        self.p2_left, self.p3_left = self.p2_center, self.p3_center
+        self.p1_bottom, self.p3_bottom = self.p1_center, self.p3_center
+        self.p1_back, self.p2_back = self.p1_center, self.p2_center
+        self.dp1 = torch.tensor(dp[0], dtype=F64, device=dev).reshape(1, S, 1, 1)
+        self.dp2 = torch.tensor(dp[1], dtype=F64, device=dev).reshape(1, S, 1, 1)
+        self.dp3 = torch.tensor(dp[2], dtype=F64, device=dev).reshape(1, S, 1, 1)
+        # collision time: params.tau(q1, q2, p1, p2, p3) must be uniform
+        self._tau = 1.0
+        if self._source_on:
+            tau = self.params.tau(self.q1_center, self.q2_center, self.p1_center,
+                                  self.p2_center, self.p3_center)
+            tau = torch.as_tensor(tau, dtype=F64, device=dev)
+            t0 = float(tau.reshape(-1)[0])
+            if not bool((tau == t0).all()):
+                raise NotImplementedError('the fused kernel needs a uniform tau')
+            self._tau = t0
+
+    # -- layout helpers -------------------------------------------------------
+    def _q_interior(self, x):
+        """(..., Q1g, Q2g) -> interior (..., N_q1, N_q2)."""
+        g1, g2 = self.N_g1, self.N_g2
+        return x[..., g1:g1 + self.N_q1, g2:g2 + self.N_q2]
+
+    def _q_padded(self, flat):
+        """[Q] (index i1 + N_q1 i2) -> (1, 1, Q1g, Q2g) with periodic ghosts."""
+        a = flat.reshape(self.N_q2, self.N_q1).t()
+        i1 = (torch.arange(-self.N_g1, self.N_q1 + self.N_g1, device=a.device)) % self.N_q1
+        i2 = (torch.arange(-self.N_g2, self.N_q2 + self.N_g2, device=a.device)) % self.N_q2
+        return a[i1][:, i2].reshape(1, 1, len(i1), len(i2))
+
+    def _to_internal(self, f_ref):
+        """(P, S, Q1g, Q2g) -> contiguous [N_q2][N_q1][P] in the storage dtype."""
+        f_ref = torch.as_tensor(f_ref, device=self.device)
+        f_ref = torch.broadcast_to(f_ref, (self._P, 1) + tuple(self.q1_center.shape[2:]))
+        inner = self._q_interior(f_ref)[:, 0]
+        return inner.permute(2, 1, 0).contiguous().to(self._dtype)
+
+    def _to_reference(self, f_int):
+        """[N_q2][N_q1][P] -> (P, 1, Q1g, Q2g) with periodic ghost zones."""
+        i1 = torch.arange(-self.N_g1, self.N_q1 + self.N_g1, device=f_int.device) % self.N_q1
+        i2 = torch.arange(-self.N_g2, self.N_q2 + self.N_g2, device=f_int.device) % self.N_q2
+        padded = f_int[i2][:, i1]
+        return padded.permute(2, 1, 0).unsqueeze(1)
+
+    @property
+    def f(self):
+        """The distribution function in the reference's (P, S, Q1g, Q2g)
+        layout, ghost zones filled periodically (a copy)."""
+        return self._to_reference(self._f)
+
+    @f.setter
+    def f(self, value):
+        self._f = self._to_internal(value)
+        self._update_moments()
+
+    # -- initialisation (nonlinear_solver.py:464-500) -------------------------
+    def _initialize(self, params):
+        f = self.physical_system.initial_conditions.initialize_f(
+            self.q1_center, self.q2_center, self.p1_center, self.p2_center,
+            self.p3_center, params)
+        self._f = self._to_internal(torch.as_tensor(f, dtype=F64, device=self.device))
+        self.f_initial = self._f.clone()
+        self._update_moments()
+        self.fields_solver = _fields_solver(self) if self._fields_on else None
+
+    def _update_moments(self):
+        _lib.check(self._L.boltx_moments(ctypes.byref(self._grid), self._precision,
+                                         _lib.ptr(self._f), _lib.ptr(self._mom),
+                                         _lib.stream_handle()))
+
+    def _buf(self, name, dtype=None, shape=None):
+        dtype = dtype or self._dtype
+        shape = shape or (self.N_q2, self.N_q1, self._P)
+        t = self._bufs.get(name)
+        if t is None or t.dtype != dtype or tuple(t.shape) != tuple(shape):
+            t = torch.empty(shape, dtype=dtype, device=self.device)
+            self._bufs[name] = t
+        return t
+
+    # -- the step -------------------------------------------------------------
+    def _stage(self, u, w, un, out, a, b, cdt, mom_in, mom_out):
+        """One RK stage, out = a*un + b*(w + cdt*L(u)) (df_dt_fvm.py:33-339
+        inside fvm_operator.py:22-33)."""
+        F = [None, None, None]
+        if self._p_adv:
+            fs = self.fields_solver
+            if fs.at_n is False:  # df_dt_fvm.py:102-104, 214-233
+                J = self._buf('J', F64, (3, self._Q))
+                _lib.check(self._L.boltx_currents(ctypes.byref(self._grid),
+                                                  ctypes.byref(self._phys), _lib.ptr(u),
+                                                  _lib.ptr(J), _lib.stream_handle()))
+                fs.evolve_electrodynamic_fields(J, self.dt)
+            F = [fs._take_for_C_p() for _ in range(3)]  # df_dt_fvm.py:251-267
+        st = _lib.Stage(_lib.ptr(u), _lib.ptr(w), _lib.ptr(un), _lib.ptr(out),
+                        float(a), float(b), float(cdt), _lib.ptr(mom_in), _lib.ptr(mom_out),
+                        _lib.ptr(F[0]), _lib.ptr(F[1]), _lib.ptr(F[2]))
+        _lib.check(self._L.boltx_fvm_stage(ctypes.byref(self._grid), ctypes.byref(self._phys),
+                                           ctypes.byref(st), _lib.stream_handle()))
+
+    def _advance(self, dt):
+        """Advance f by dt without the host-side divergence check (the device
+        flag is still set; see strang_timestep)."""
+        self.dt = dt
+        f = self._f
+        if self._stepper == 'RK2':
+            # fvm_operator.py:25-33
+            u1, m1 = self._buf('u1'), self._buf('m1', F64, (5, self._Q))
+            self._stage(f, f, None, u1, 0.0, 1.0, dt / 2, self._mom, m1)
+            self._stage(u1, f, None, f, 0.0, 1.0, dt, m1, self._mom)
+        else:
+            u1, m1 = self._buf('u1'), self._buf('m1', F64, (5, self._Q))
+            u2, m2 = self._buf('u2'), self._buf('m2', F64, (5, self._Q))
+            self._stage(f, f, None, u1, 0.0, 1.0, dt, self._mom, m1)
+            self._stage(u1, u1, f, u2, 0.75, 0.25, dt, m1, m2)
+            self._stage(u2, u2, f, f, 1 / 3, 2 / 3, dt, m2, self._mom)
+        _lib.check(self._L.boltx_check_finite(ctypes.byref(self._grid), _lib.ptr(self._mom),
+                                              _lib.ptr(self._flag), _lib.stream_handle()))
+        self.time_elapsed += dt
+
+    def check_divergence(self):
+        """timestep.py:28-38 (reads the device flag: one host sync)."""
+        if int(self._flag.item()) != 0:
+            raise SystemExit('Solver Diverging!')
+
+    def strang_timestep(self, dt):
+        """timestep.py:101-160: FVM in q and p needs no splitting."""
+        if self.performance_test_flag:
+            t0 = torch.cuda.Event(enable_timing=True)
+            t1 = torch.cuda.Event(enable_timing=True)
+            t0.record()
+        self._advance(dt)
+        self.check_divergence()
+        if self.performance_test_flag:
+            t1.record()
+            t1.synchronize()
+            self._timings['time_ts'] += t0.elapsed_time(t1) / 1e3
+            self._timings['steps'] += 1
+
+    lie_timestep = strang_timestep
+    swss_timestep = strang_timestep
+    jia_timestep = strang_timestep
+
+    # -- moments (compute_moments.py:7-59) -----------------------------------
+    def compute_moments(self, moment_name, f=None):
+        """Moments shaped (1, N_s, Q1g, Q2g).  The built-in five come from the
+        HIP reduction; other moment functions of the user's module are
+        evaluated with torch on the device."""
+        if f is None and moment_name in _mom.BUILTIN and \
+                self.physical_system.moments is _mom:
+            return self._q_padded(self._mom[_mom.BUILTIN.index(moment_name)])
+        fn = getattr(self.physical_system.moments, moment_name)
+        ff = self.f if f is None else torch.as_tensor(f, device=self.device)
+        return fn(ff.to(F64), self.p1_center, self.p2_center, self.p3_center,
+                  self.dp3 * self.dp2 * self.dp1)
+
+    def print_performance_timings(self, N_iters=None):
+        """performance_timings.py:9-165 (reduced: one GPU, one table row)."""
+        n = self._timings['steps']
+        if n == 0:
+            print('no timed steps (construct with performance_test_flag=True)')
+            return
+        t = self._timings['time_ts'] / n
+        cells = self._Q * self._P
+        print('Time per step           : %.6e s' % t)
+        print('Phase-space cell-updates/s: %.6e' % (cells / t))
+        print('Spatial Zone Cycles/s   : %.6e' % (self._Q / t))

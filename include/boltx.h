@@ -1,0 +1,130 @@
+/* C-ABI of the MI355X (gfx950) nonlinear FVM time step.
+ *
+ * The reference (QuazarTech/Bolt) is pure Python over ArrayFire; it has no FFI.
+ * Each entry point below replaces the whole-array ArrayFire work of one
+ * reference function (cited per function), and is called only by the Python
+ * mirror of the reference interface (bolt_amd.nonlinear_solver), never by user
+ * code.  Conventions:
+ *   - plain C: sizes, spacings and device pointers; no allocation of
+ *     persistent memory inside the library;
+ *   - every call is asynchronous on the given HIP stream (a hipStream_t passed
+ *     as void*, NULL = the null stream) and re-entrant;
+ *   - return 0 on success, a negative boltx_status otherwise; the message is
+ *     in boltx_last_error() (thread-local).
+ *
+ * Layout of a distribution function: interior q-cells only (periodic q is
+ * handled by modular indexing, so there are no ghost zones), C-order
+ * [n_q2][n_q1][P] with P = n_p1*n_p2*n_p3 and p-index i1 + n_p1*(i2 + n_p2*i3)
+ * (p1 fastest), i.e. the byte order of the reference's (P, S, Q1, Q2)
+ * column-major af.Array for S = 1 without its ghost rows.  FP64 storage is
+ * double, FP32-bulk storage is float.
+ * Moments are FP64 arrays [5][Q] (Q = n_q1*n_q2): density, mom_v1_bulk,
+ * mom_v2_bulk, mom_v3_bulk, energy, each already multiplied by dp3*dp2*dp1.
+ * Cell-centred fields are FP64 arrays [6][Q]: E1, E2, E3, B1, B2, B3.
+ */
+#ifndef BOLTX_H
+#define BOLTX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BOLTX_VERSION 1
+
+enum boltx_recon {
+  BOLTX_RECON_PIECEWISE_CONSTANT = 0,
+  BOLTX_RECON_MINMOD = 1,
+  BOLTX_RECON_PPM = 2,
+  BOLTX_RECON_WENO5 = 3
+};
+enum boltx_riemann { BOLTX_RIEMANN_UPWIND = 0, BOLTX_RIEMANN_LLF = 1 };
+enum boltx_precision { BOLTX_FP64 = 0, BOLTX_FP32_BULK = 1 };
+enum boltx_status {
+  BOLTX_OK = 0,
+  BOLTX_EINVAL = -1,
+  BOLTX_ELAUNCH = -2,
+  BOLTX_EUNSUPPORTED = -3
+};
+
+typedef struct boltx_grid {
+  int32_t n_q1, n_q2;       /* interior q-cells, periodic */
+  int32_t n_p1, n_p2, n_p3; /* p-cells per axis */
+  int32_t n_species;        /* 1 */
+  double dq1, dq2;
+  double dp1, dp2, dp3;
+  double p1_start, p2_start, p3_start;
+} boltx_grid;
+
+typedef struct boltx_physics {
+  int32_t recon_q, recon_p;     /* enum boltx_recon */
+  int32_t riemann_q, riemann_p; /* enum boltx_riemann */
+  int32_t source_enabled;       /* BGK collision source on */
+  int32_t p_advection;          /* FVM advection in p (fields enabled) */
+  int32_t p_dim;                /* 1, 2 or 3 */
+  int32_t precision;            /* enum boltx_precision */
+  double mass, charge, boltzmann_constant;
+  double tau;                   /* constant collision time */
+} boltx_physics;
+
+typedef struct boltx_stage {
+  const void *u;   /* stage input, read through the stencils */
+  const void *w;   /* base of the update, read pointwise (may equal u or un) */
+  const void *un;  /* step-start state, read pointwise when a != 0 */
+  void *out;       /* stage output; must not alias u (may alias w or un) */
+  double a, b, cdt; /* out = a*un + b*(w + cdt*L(u)) */
+  const double *mom_in;  /* [5][Q] moments of u (needed when the source is on) */
+  double *mom_out;       /* [5][Q] moments of out, or NULL */
+  const double *fields_c_p1; /* [6][Q] fields seen by C_p1 (p_advection) */
+  const double *fields_c_p2; /* [6][Q] fields seen by C_p2 */
+  const double *fields_c_p3; /* [6][Q] fields seen by C_p3 */
+} boltx_stage;
+
+/* One Runge-Kutta stage of the FVM step: out = a*un + b*(w + cdt*L(u)) where
+ * L(u) = -div_q(C_q u) + BGK(u) - div_p(C_p u) with the reference's
+ * reconstruction and upwind / LLF fluxes, fused with the moments of out.
+ * Replaces df_dt_fvm (bolt/lib/nonlinear/finite_volume/df_dt_fvm.py:33-339)
+ * together with one stage of timestep_fvm (fvm_operator.py:22-33) and the
+ * BGK source (bolt/src/nonrelativistic_boltzmann/collision_operator.py:70-128). */
+int boltx_fvm_stage(const boltx_grid *grid, const boltx_physics *phys,
+                    const boltx_stage *stage, void *stream);
+
+/* The five built-in moments of f, [5][Q] FP64.  Replaces compute_moments
+ * (bolt/lib/nonlinear/compute_moments.py:7-59) for density, mom_v1_bulk,
+ * mom_v2_bulk, mom_v3_bulk and energy (moments.py:5-22). */
+int boltx_moments(const boltx_grid *grid, int32_t precision, const void *f,
+                  double *mom, void *stream);
+
+/* Currents J1, J2, J3 ([3][Q] FP64) from the upwind face states of u:
+ * J1 = charge * mom_v1_bulk(f_left), J2 = charge * mom_v2_bulk(f_bot),
+ * J3 = charge * mom_v3_bulk(f).  Replaces df_dt_fvm.py:214-224. */
+int boltx_currents(const boltx_grid *grid, const boltx_physics *phys,
+                   const void *u, double *J, void *stream);
+
+/* One FDTD leapfrog on the periodic Yee grid, in place on yee ([6][Q] FP64:
+ * E1(i,j+1/2), E2(i+1/2,j), E3(i+1/2,j+1/2), B1(i+1/2,j), B2(i,j+1/2),
+ * B3(i,j)): E += dt/(mu eps) curl B - dt J / eps, then B -= dt curl E.
+ * Replaces fdtd (bolt/lib/nonlinear/fields/electrodynamic/fdtd_explicit.py:141-177). */
+int boltx_fdtd_step(const boltx_grid *grid, double *yee, const double *J,
+                    double dt, double eps, double mu, void *stream);
+
+/* Yee grid -> cell centres ([6][Q] each).  Replaces
+ * yee_grid_to_cell_centered_grid (bolt/lib/nonlinear/fields/fields.py:527-557). */
+int boltx_yee_to_cell_centered(const boltx_grid *grid, const double *yee,
+                               double *cc, void *stream);
+
+/* Sets *flag (device int) to 1 if any moment density is not finite; a
+ * non-finite f makes its density sum non-finite.  Replaces check_divergence
+ * (bolt/lib/nonlinear/timestep.py:28-38) without a host sync. */
+int boltx_check_finite(const boltx_grid *grid, const double *mom, int32_t *flag,
+                       void *stream);
+
+int boltx_version(void);
+const char *boltx_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BOLTX_H */

@@ -1,0 +1,155 @@
+"""Problem definitions shared by the tests, the smoke check and the bench.
+
+Each case is built as the reference's user modules (domain, boundary
+conditions, params, initialize) with torch callables, so the same modules drive
+the oracle (CPU tensors) and bolt_amd (GPU tensors).  Workloads follow the
+synthetic Maxwellian-perturbation problems: 1D1V minmod RK2, 2D2V WENO5 SSP-RK3,
+2D3V electromagnetic, at sizes the oracle finishes in seconds unless stated.
+"""
+import math
+import types
+
+import numpy as np
+import torch
+
+
+def _module(name, **attrs):
+    m = types.ModuleType(name)
+    for k, v in attrs.items():
+        setattr(m, k, v)
+    return m
+
+
+def _const_tau(value):
+    def tau(q1, q2, p1, p2, p3):
+        return value * (q1 * 0 + 1) * (p1 * 0 + 1)
+    return tau
+
+
+def maxwellian_case(*, N_q1, N_q2, N_p1, N_p2=1, N_p3=1, N_ghost=3, p_max=10.0,
+                    p_dim=1, modes=((1, 0),), amps=(1e-3,), phases=(0.0,),
+                    recon_q="weno5", recon_p="weno5", riemann="upwind-flux",
+                    stepper="SSP-RK3", tau=0.01, source=True, fields=False,
+                    B0=0.5, charge=-1.0, mass=1.0, precision="fp64",
+                    q2_bc="periodic", p3_max=None):
+    """Maxwellian (T = 1, u = 0) with density n = 1 + sum A_k cos(2 pi (kx q1 +
+    ky q2) + phi_k) on [0,1)^2, p in [-p_max, p_max] per active axis (singleton
+    axes span [-0.5, 0.5] as in the reference examples)."""
+    def ext(N, pm):
+        return ([-0.5], [0.5]) if N == 1 else ([-pm], [pm])
+    p1s, p1e = ext(N_p1, p_max)
+    p2s, p2e = ext(N_p2, p_max)
+    p3s, p3e = ext(N_p3, p3_max or p_max)
+    domain = _module("domain", q1_start=0.0, q1_end=1.0, N_q1=N_q1,
+                     q2_start=0.0, q2_end=1.0, N_q2=N_q2,
+                     p1_start=p1s, p1_end=p1e, N_p1=N_p1,
+                     p2_start=p2s, p2_end=p2e, N_p2=N_p2,
+                     p3_start=p3s, p3_end=p3e, N_p3=N_p3, N_ghost=N_ghost)
+    bc = _module("boundary_conditions", in_q1_left="periodic", in_q1_right="periodic",
+                 in_q2_bottom=q2_bc, in_q2_top=q2_bc)
+    params = _module(
+        "params", fields_type="electrodynamic", fields_initialize="user-defined",
+        fields_solver="fdtd", solver_method_in_q="FVM", solver_method_in_p="FVM",
+        reconstruction_method_in_q=recon_q, reconstruction_method_in_p=recon_p,
+        riemann_solver_in_q=riemann, riemann_solver_in_p=riemann, p_dim=p_dim,
+        num_devices=1, mass=[mass], boltzmann_constant=1, charge=[charge],
+        fields_enabled=fields, source_enabled=source, instantaneous_collisions=False,
+        hybrid_model_enabled=False, eps=1, mu=1, tau=_const_tau(tau),
+        time_stepper_fvm=stepper, precision=precision, B0=B0,
+        modes=tuple(modes), amps=tuple(amps), phases=tuple(phases))
+
+    def initialize_f(q1, q2, p1, p2, p3, params):
+        n = 1.0 + 0.0 * q1
+        for (kx, ky), A, ph in zip(params.modes, params.amps, params.phases):
+            n = n + A * torch.cos(2 * math.pi * (kx * q1 + ky * q2) + ph)
+        m, k, T = params.mass, params.boltzmann_constant, 1.0
+        f = n * (m / (2 * math.pi * k * T)) ** (params.p_dim / 2) \
+            * torch.exp(-m * p1**2 / (2 * k * T))
+        if params.p_dim >= 2:
+            f = f * torch.exp(-m * p2**2 / (2 * k * T))
+        if params.p_dim >= 3:
+            f = f * torch.exp(-m * p3**2 / (2 * k * T))
+        return f
+
+    def initialize_E(q1, q2, params):
+        z = 0.0 * q1
+        return (z, z, z)
+
+    def initialize_B(q1, q2, params):
+        z = 0.0 * q1
+        return (z, z, z + params.B0)
+
+    init = _module("initialize", initialize_f=initialize_f, initialize_E=initialize_E,
+                   initialize_B=initialize_B)
+    return types.SimpleNamespace(domain=domain, boundary_conditions=bc, params=params,
+                                 initialize=init)
+
+
+def draw(seed, lo, hi, n=None):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(lo, hi, n)
+
+
+def case_c1(seed=0, **over):
+    """1D1V, fields off, minmod, upwind, BGK tau = 0.01, RK2, dt = 1e-3:
+    N_q1 = 32 (2 ghosts), N_p1 = 32 on [-10, 10]."""
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(1e-4, 1e-3)
+    phi = rng.uniform(0, 2 * math.pi)
+    kw = dict(N_q1=32, N_q2=1, N_p1=32, N_ghost=2, p_max=10.0, p_dim=1,
+              modes=((1, 0),), amps=(A,), phases=(phi,), recon_q="minmod",
+              recon_p="minmod", riemann="upwind-flux", stepper="RK2", tau=0.01)
+    kw.update(over)
+    c = maxwellian_case(**kw)
+    c.dt, c.steps = 1e-3, 100
+    return c
+
+
+def case_2d2v(seed=0, N_q=128, N_p=48, **over):
+    """2D2V, fields off, WENO5, LLF, BGK tau = 1e-2, SSP-RK3, dt = 5e-4: the
+    full size is N_q = 128, N_p = 48 on [-10, 10]^2 (3 ghosts)."""
+    rng = np.random.default_rng(seed)
+    modes = ((1, 1), (1, 2), (2, 1), (2, 2))
+    A = rng.uniform(1e-4, 1e-3, 4)
+    phi = rng.uniform(0, 2 * math.pi, 4)
+    kw = dict(N_q1=N_q, N_q2=N_q, N_p1=N_p, N_p2=N_p, N_ghost=3, p_max=10.0, p_dim=2,
+              modes=modes, amps=tuple(A), phases=tuple(phi), recon_q="weno5",
+              recon_p="weno5", riemann="lax-friedrichs", stepper="SSP-RK3", tau=1e-2)
+    kw.update(over)
+    c = maxwellian_case(**kw)
+    c.dt, c.steps = 5e-4, 200
+    return c
+
+
+def case_2d3v(seed=0, N_q=64, N_p=24, **over):
+    """2D3V electromagnetic: electrons (q/m = -1), PPM in q, WENO5 in p, LLF,
+    BGK tau = 0.1, p in [-8, 8]^3, B = (0, 0, 0.5), E = 0, dt = 2e-4."""
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(1e-3, 1e-2)
+    phi = rng.uniform(0, 2 * math.pi)
+    kw = dict(N_q1=N_q, N_q2=N_q, N_p1=N_p, N_p2=N_p, N_p3=N_p, N_ghost=3, p_max=8.0,
+              p_dim=3, modes=((1, 1),), amps=(A,), phases=(phi,), recon_q="ppm",
+              recon_p="weno5", riemann="lax-friedrichs", stepper="SSP-RK3", tau=0.1,
+              fields=True, B0=0.5, charge=-1.0)
+    kw.update(over)
+    c = maxwellian_case(**kw)
+    c.dt, c.steps = 2e-4, 500
+    return c
+
+
+def oracle_solver(case):
+    from oracle import bolt_oracle as bo
+    sysm = bo.make_system(case.domain, case.boundary_conditions, case.params, case.initialize,
+                          bo.advection_terms, bo.BGK, bo.moments)
+    return bo.OracleSolver(sysm)
+
+
+def hip_solver(case):
+    from bolt_amd.physical_system import physical_system
+    from bolt_amd.nonlinear_solver import nonlinear_solver
+    import bolt_amd.src.nonrelativistic_boltzmann.advection_terms as adv
+    import bolt_amd.src.nonrelativistic_boltzmann.collision_operator as coll
+    import bolt_amd.src.nonrelativistic_boltzmann.moments as mom
+    system = physical_system(case.domain, case.boundary_conditions, case.params,
+                             case.initialize, adv, coll.BGK, mom)
+    return nonlinear_solver(system)

@@ -1,0 +1,101 @@
+// CPU harness for the tests: runs the stage kernel's per-cell code
+// (bolt_amd/csrc/fvm_stage.h) on host memory, one q-cell after another, so
+// the kernel arithmetic and indexing can be compared with the oracle without
+// a GPU.  Built host-only by tests/hostcheck/build_host.py; never shipped.
+#include <cstring>
+#include <vector>
+
+#include "../../include/boltx.h"
+#include "../../bolt_amd/csrc/fvm_stage.h"
+
+using namespace bx;
+
+static StageParams params_of(const boltx_grid* g, const boltx_physics* ph) {
+  StageParams sp;
+  memset(&sp, 0, sizeof(sp));
+  sp.n_q1 = g->n_q1; sp.n_q2 = g->n_q2;
+  sp.n_p1 = g->n_p1; sp.n_p2 = g->n_p2; sp.n_p3 = g->n_p3;
+  sp.P = g->n_p1 * g->n_p2 * g->n_p3;
+  sp.Q = g->n_q1 * g->n_q2;
+  sp.p1s = g->p1_start; sp.p2s = g->p2_start; sp.p3s = g->p3_start;
+  sp.dp1 = g->dp1; sp.dp2 = g->dp2; sp.dp3 = g->dp3;
+  sp.inv_dq1 = 1.0 / g->dq1; sp.inv_dq2 = 1.0 / g->dq2;
+  sp.inv_dp1 = 1.0 / g->dp1; sp.inv_dp2 = 1.0 / g->dp2; sp.inv_dp3 = 1.0 / g->dp3;
+  sp.measure = g->dp3 * g->dp2 * g->dp1;
+  if (ph) {
+    sp.riemann_q = ph->riemann_q; sp.riemann_p = ph->riemann_p;
+    sp.source = ph->source_enabled; sp.p_dim = ph->p_dim;
+    sp.mass = ph->mass; sp.charge = ph->charge; sp.kB = ph->boltzmann_constant;
+    sp.inv_tau = ph->source_enabled ? 1.0 / ph->tau : 0.0;
+  }
+  return sp;
+}
+
+template <typename T, int RQ, int RP, bool PADV>
+static void run(StageParams sp) {
+  std::vector<T> tab(sp.n_p1 + sp.n_p2 + sp.n_p3);
+  for (int q = 0; q < sp.Q; ++q) {
+    QCell<T> c;
+    qcell_neighbours(sp, q, c);
+    c.pref = 0.0;
+    c.e1 = c.e2 = c.e3 = tab.data();
+    if (sp.source) {
+      const BgkCell b = qcell_bgk(sp, q);
+      c.pref = maxwell_prefactor(b, sp.mass, sp.kB, sp.p_dim);
+      for (int i = 0; i < sp.n_p1; ++i)
+        tab[i] = T(maxwell_factor(centre(sp.p1s, i, sp.dp1), b.u1, b.T, sp.mass, sp.kB));
+      for (int i = 0; i < sp.n_p2; ++i)
+        tab[sp.n_p1 + i] = T(maxwell_factor(centre(sp.p2s, i, sp.dp2), b.u2, b.T, sp.mass, sp.kB));
+      for (int i = 0; i < sp.n_p3; ++i)
+        tab[sp.n_p1 + sp.n_p2 + i] =
+            T(maxwell_factor(centre(sp.p3s, i, sp.dp3), b.u3, b.T, sp.mass, sp.kB));
+      c.e1 = tab.data();
+      c.e2 = tab.data() + sp.n_p1;
+      c.e3 = tab.data() + sp.n_p1 + sp.n_p2;
+    }
+    const T* blk = (const T*)sp.u + (long)q * sp.P;
+    if (PADV) qcell_fields(sp, c);
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int p = 0; p < sp.P; ++p) {
+      double v[3];
+      const T x = stage_cell<T, RQ, RP, PADV>(sp, c, p, blk, v);
+      ((T*)sp.out)[(long)q * sp.P + p] = x;
+      acc[0] += x; acc[1] += x * v[0]; acc[2] += x * v[1]; acc[3] += x * v[2];
+      acc[4] += (0.5 * x) * ((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+    }
+    if (sp.mom_out)
+      for (int k = 0; k < 5; ++k) sp.mom_out[k * sp.Q + q] = acc[k] * sp.measure;
+  }
+}
+
+template <typename T, int RQ>
+static void run_rp(StageParams sp, int rp, bool padv) {
+  if (!padv) return run<T, RQ, RC_PC, false>(sp);
+  switch (rp) {
+    case 0: return run<T, RQ, RC_PC, true>(sp);
+    case 1: return run<T, RQ, RC_MINMOD, true>(sp);
+    case 2: return run<T, RQ, RC_PPM, true>(sp);
+    default: return run<T, RQ, RC_WENO5, true>(sp);
+  }
+}
+
+template <typename T>
+static void run_rq(StageParams sp, int rq, int rp, bool padv) {
+  switch (rq) {
+    case 0: return run_rp<T, RC_PC>(sp, rp, padv);
+    case 1: return run_rp<T, RC_MINMOD>(sp, rp, padv);
+    case 2: return run_rp<T, RC_PPM>(sp, rp, padv);
+    default: return run_rp<T, RC_WENO5>(sp, rp, padv);
+  }
+}
+
+extern "C" int host_fvm_stage(const boltx_grid* g, const boltx_physics* ph, const boltx_stage* st) {
+  StageParams sp = params_of(g, ph);
+  sp.a = st->a; sp.b = st->b; sp.cdt = st->cdt;
+  sp.u = st->u; sp.w = st->w; sp.un = st->un; sp.out = st->out;
+  sp.mom_in = st->mom_in; sp.mom_out = st->mom_out;
+  sp.fp1 = st->fields_c_p1; sp.fp2 = st->fields_c_p2; sp.fp3 = st->fields_c_p3;
+  if (ph->precision == 0) run_rq<double>(sp, ph->recon_q, ph->recon_p, ph->p_advection != 0);
+  else run_rq<float>(sp, ph->recon_q, ph->recon_p, ph->p_advection != 0);
+  return 0;
+}

@@ -1,0 +1,119 @@
+"""The stage kernel's per-cell code (bolt_amd/csrc/fvm_stage.h), run on the
+CPU through tests/hostcheck, against the FP64 oracle: one RK stage from the
+same state.  Tolerances: FP64 <= 1e-12 relative L-inf of f (the per-step
+parity bar); FP32-bulk <= 1e-5 relative L2 against the FP64 oracle."""
+import numpy as np
+import pytest
+
+from tests.cases import case_2d2v, case_c1, case_2d3v, oracle_solver
+from tests.hostcheck import harness as H
+
+RECON = ["piecewise-constant", "minmod", "ppm", "weno5"]
+RIEMANN = ["upwind-flux", "lax-friedrichs"]
+
+
+def _one_stage(o, dt):
+    o.dt = dt
+    o._communicate_f()
+    un = o.f
+    L = o.df_dt_fvm(un)
+    return un, un + L * dt
+
+
+def _rel(a, b):
+    d = np.abs(a.astype(np.float64) - b)
+    return d.max() / np.abs(b).max(), np.sqrt((d**2).sum() / (b**2).sum())
+
+
+@pytest.mark.parametrize("rq", RECON)
+@pytest.mark.parametrize("rs", RIEMANN)
+def test_stage_2d2v_fp64(rq, rs):
+    c = case_2d2v(seed=1, N_q=12, N_p=10, recon_q=rq, riemann=rs)
+    o = oracle_solver(c)
+    un, ref = _one_stage(o, c.dt)
+    u = H.to_internal(o, un)
+    out, mom = H.stage(o, u, u, None, 0.0, 1.0, c.dt)
+    linf, _ = _rel(out, H.to_internal(o, ref))
+    assert linf <= 1e-12
+    # the fused moments of the output equal the oracle's moments of the output
+    m_ref = H.moments_np(o, H.to_internal(o, ref))
+    assert np.abs(mom - m_ref).max() <= 1e-13 * np.abs(m_ref).max()
+
+
+def test_stage_weno5_smooth_full_q_resolution():
+    """At 128^2 the data is so smooth that the expanded WENO indicator is
+    rounding-dominated; the literal evaluation keeps parity."""
+    c = case_2d2v(seed=2, N_q=128, N_p=6)
+    o = oracle_solver(c)
+    un, ref = _one_stage(o, c.dt)
+    u = H.to_internal(o, un)
+    out, _ = H.stage(o, u, u, None, 0.0, 1.0, c.dt)
+    linf, _ = _rel(out, H.to_internal(o, ref))
+    assert linf <= 1e-12
+
+
+@pytest.mark.parametrize("rq", RECON)
+def test_stage_fp32_bulk(rq):
+    c = case_2d2v(seed=3, N_q=16, N_p=12, recon_q=rq)
+    o = oracle_solver(c)
+    un, ref = _one_stage(o, c.dt)
+    u = H.to_internal(o, un)
+    out, _ = H.stage(o, u, u, None, 0.0, 1.0, c.dt, precision=1)
+    _, l2 = _rel(out, H.to_internal(o, ref))
+    assert l2 <= 1e-6
+
+
+def test_rk_combination_matches_oracle():
+    """out = a*un + b*(w + cdt*L(u)) for the SSP-RK3 second stage."""
+    c = case_2d2v(seed=4, N_q=10, N_p=8)
+    o = oracle_solver(c)
+    o.dt = c.dt
+    o._communicate_f()
+    un = o.f
+    u1 = un + o.df_dt_fvm(un) * c.dt
+    o.f = u1
+    o._communicate_f()
+    u1 = o.f
+    ref = 0.75 * un + 0.25 * (u1 + o.df_dt_fvm(u1) * c.dt)
+    a1, an = H.to_internal(o, u1), H.to_internal(o, un)
+    out, _ = H.stage(o, a1, a1, an, 0.75, 0.25, c.dt)
+    assert _rel(out, H.to_internal(o, ref))[0] <= 1e-12
+
+
+def test_stage_1d1v_singleton_q2():
+    c = case_c1(seed=0)
+    o = oracle_solver(c)
+    o.dt = c.dt
+    o._communicate_f()
+    un = o.f
+    ref = un + o.df_dt_fvm(un) * (c.dt / 2)
+    u = H.to_internal(o, un)
+    out, _ = H.stage(o, u, u, None, 0.0, 1.0, c.dt / 2)
+    assert _rel(out, H.to_internal(o, ref))[0] <= 1e-14
+
+
+@pytest.mark.parametrize("rp", RECON)
+def test_stage_fields_p_advection(rp):
+    """Fields on, first RK stage (at_n True: no FDTD); C_p1, C_p2, C_p3 read
+    the aliased cell-centred fields."""
+    c = case_2d3v(seed=5, N_q=6, N_p=8, recon_p=rp, B0=0.5)
+    o = oracle_solver(c)
+    # give the fields some structure so every C_p term matters
+    fs = o.fields_solver
+    g = torch_rand_like(fs.cell_centered_EM_fields, seed=9)
+    fs.cell_centered_EM_fields[:] = g  # mutate the shared (aliased) object
+    o.dt = c.dt
+    o._communicate_f()
+    un = o.f
+    ref = un + o.df_dt_fvm(un) * c.dt
+    u = H.to_internal(o, un)
+    cc = fs.cell_centered_EM_fields[:, 0, o.N_g1:o.N_g1 + o.N_q1, o.N_g2:o.N_g2 + o.N_q2]
+    F = np.ascontiguousarray(cc.permute(0, 2, 1).reshape(6, -1).numpy())
+    out, _ = H.stage(o, u, u, None, 0.0, 1.0, c.dt, fields=[F, F, F])
+    assert _rel(out, H.to_internal(o, ref))[0] <= 1e-12
+
+
+def torch_rand_like(t, seed):
+    import torch
+    g = torch.Generator().manual_seed(seed)
+    return torch.rand(t.shape, generator=g, dtype=t.dtype) * 0.4 - 0.2
