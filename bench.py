@@ -102,7 +102,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the step shards over q with halo exchange; see DESIGN.md)", file=sys.stderr)
+              "(the step shards over q with a halo exchange per stage)", file=sys.stderr)
         sys.exit(2)
     import torch
     nls64, r64 = run_gpu('fp64', args.steps, args.warmup)

@@ -48,10 +48,13 @@ def test_c1_per_step_and_full_run(gpu):
     linf, l2 = _cmp(nls, o)
     print(f"C1 100 steps: rel Linf {linf:.3e} rel L2 {l2:.3e}", flush=True)
     assert linf <= 1e-11
+    g1, g2 = nls.N_g1, nls.N_g2
     for name in ("density", "mom_v1_bulk", "energy"):
-        a = nls.compute_moments(name).cpu().numpy()
-        b = o.compute_moments(name).numpy()
-        assert np.abs(a - b).max() <= 1e-11 * np.abs(b).max()
+        # interior only: the reference's ghost zones are stale after a step
+        a = nls.compute_moments(name).cpu().numpy()[..., g1:g1 + nls.N_q1, g2:g2 + nls.N_q2]
+        b = o.compute_moments(name).numpy()[..., g1:g1 + nls.N_q1, g2:g2 + nls.N_q2]
+        err = float(np.abs(a - b).max() / np.abs(b).max())
+        assert err <= 1e-11, (name, err)
 
 
 @pytest.mark.parametrize("rq", ["piecewise-constant", "minmod", "ppm", "weno5"])
@@ -67,10 +70,44 @@ def test_2d2v_rk2_and_rk3(gpu):
         _per_step(c, 3, 1e-12)
 
 
+def _oracle_self_sensitivity(case):
+    """One oracle step from u and from u with 1-ulp random noise: the
+    reference algorithm's own conditioning."""
+    o1, o2 = oracle_solver(case), oracle_solver(case)
+    g = torch.Generator().manual_seed(0)
+    o2.f = o2.f * (1 + torch.randint(-1, 2, o2.f.shape, generator=g).double() * 2.0**-52)
+    o1.strang_timestep(case.dt)
+    o2.strang_timestep(case.dt)
+    a, b = o1.interior(), o2.interior()
+    return float((a - b).abs().max() / b.abs().max())
+
+
 def test_2d2v_weno5_fine_q(gpu):
-    """Full q-resolution (128^2) where the WENO5 indicator is rounding-dominated."""
+    """Full q-resolution (128^2), where the expanded WENO5 indicator is
+    rounding-dominated: one RHS evaluation from the same state agrees to
+    1e-12; a full SSP-RK3 step agrees to within 4x the oracle's own change
+    under 1-ulp input noise (stages 2 and 3 start from states that differ in
+    the last bit)."""
     c = case_2d2v(seed=3, N_q=128, N_p=6)
-    _per_step(c, 2, 1e-12)
+    o, nls = oracle_solver(c), hip_solver(c)
+    o.dt = nls.dt = c.dt
+    o._communicate_f()
+    ref = o.interior(o.f + o.df_dt_fvm(o.f) * c.dt).numpy()
+    out = torch.empty_like(nls._f)
+    m = torch.empty_like(nls._mom)
+    nls._stage(nls._f, nls._f, None, out, 0.0, 1.0, c.dt, nls._mom, m)
+    got = out.permute(2, 1, 0).unsqueeze(1).cpu().numpy()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"one RHS evaluation at 128^2: rel Linf {err:.3e}", flush=True)
+    assert err <= 1e-12
+    sens = _oracle_self_sensitivity(c)
+    o, nls = oracle_solver(c), hip_solver(c)
+    o.strang_timestep(c.dt)
+    nls.strang_timestep(c.dt)
+    linf, _ = _cmp(nls, o)
+    print(f"one step at 128^2: rel Linf {linf:.3e}; oracle 1-ulp sensitivity {sens:.3e}",
+          flush=True)
+    assert linf <= max(1e-12, 4 * sens)
 
 
 def test_fp32_bulk_error_vs_fp64_oracle(gpu):
