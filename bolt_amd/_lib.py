@@ -12,16 +12,18 @@ LIB_PATH = os.path.join(PKG, "libboltx_hip.so")
 RECON = {"piecewise-constant": 0, "minmod": 1, "ppm": 2, "weno5": 3}
 RIEMANN = {"upwind-flux": 0, "lax-friedrichs": 1}
 PRECISION = {"fp64": 0, "fp32-bulk": 1}
+LAYOUT = {"reference": 0, "q1-lines": 1}
 
-EXPORTS = ("boltx_fvm_stage", "boltx_moments", "boltx_currents", "boltx_fdtd_step",
-           "boltx_yee_to_cell_centered", "boltx_check_finite", "boltx_version",
-           "boltx_last_error")
+EXPORTS = ("boltx_fvm_stage", "boltx_moments", "boltx_workspace_size", "boltx_workspace_init",
+           "boltx_currents", "boltx_fdtd_step", "boltx_yee_to_cell_centered",
+           "boltx_check_finite", "boltx_version", "boltx_last_error")
+MOM_ROWS = 12
 
 
 class Grid(ctypes.Structure):
     _fields_ = [("n_q1", ctypes.c_int32), ("n_q2", ctypes.c_int32),
                 ("n_p1", ctypes.c_int32), ("n_p2", ctypes.c_int32), ("n_p3", ctypes.c_int32),
-                ("n_species", ctypes.c_int32),
+                ("n_species", ctypes.c_int32), ("layout", ctypes.c_int32),
                 ("dq1", ctypes.c_double), ("dq2", ctypes.c_double),
                 ("dp1", ctypes.c_double), ("dp2", ctypes.c_double), ("dp3", ctypes.c_double),
                 ("p1_start", ctypes.c_double), ("p2_start", ctypes.c_double),
@@ -43,7 +45,7 @@ class Stage(ctypes.Structure):
                 ("a", ctypes.c_double), ("b", ctypes.c_double), ("cdt", ctypes.c_double),
                 ("mom_in", ctypes.c_void_p), ("mom_out", ctypes.c_void_p),
                 ("fields_c_p1", ctypes.c_void_p), ("fields_c_p2", ctypes.c_void_p),
-                ("fields_c_p3", ctypes.c_void_p)]
+                ("fields_c_p3", ctypes.c_void_p), ("workspace", ctypes.c_void_p)]
 
 
 class BoltxError(RuntimeError):
@@ -67,13 +69,16 @@ def load(path=LIB_PATH):
     vp, i32, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_double
     g, ph, st = ctypes.POINTER(Grid), ctypes.POINTER(Physics), ctypes.POINTER(Stage)
     lib.boltx_fvm_stage.argtypes = [g, ph, st, vp]
-    lib.boltx_moments.argtypes = [g, i32, vp, vp, vp]
+    lib.boltx_moments.argtypes = [g, ph, vp, vp, vp, vp]
+    lib.boltx_workspace_size.argtypes = [g, ph]
+    lib.boltx_workspace_init.argtypes = [g, ph, vp, vp]
     lib.boltx_currents.argtypes = [g, ph, vp, vp, vp]
     lib.boltx_fdtd_step.argtypes = [g, vp, vp, dbl, dbl, dbl, vp]
     lib.boltx_yee_to_cell_centered.argtypes = [g, vp, vp, vp]
     lib.boltx_check_finite.argtypes = [g, vp, vp, vp]
     for name in EXPORTS[:-2]:
         getattr(lib, name).restype = ctypes.c_int
+    lib.boltx_workspace_size.restype = ctypes.c_size_t
     lib.boltx_version.argtypes = []
     lib.boltx_version.restype = ctypes.c_int
     lib.boltx_last_error.argtypes = []

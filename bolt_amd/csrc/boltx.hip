@@ -15,6 +15,8 @@
 
 #include "../../include/boltx.h"
 #include "fvm_stage.h"
+#include "qcell.h"
+#include "qline.h"
 
 namespace bx {
 
@@ -124,7 +126,8 @@ __global__ __launch_bounds__(kBlock) void fvm_stage_kernel(StageParams sp) {
   if (sp.mom_out) {
     block_sum(acc, red);
     if (threadIdx.x == 0) {
-      for (int k = 0; k < 5; ++k) sp.mom_out[k * sp.Q + q] = acc[k] * sp.measure;
+      for (int k = 0; k < 5; ++k) acc[k] *= sp.measure;
+      write_moment_block(sp.mom_out, sp.Q, q, acc, sp.mass, sp.kB, sp.p_dim);
     }
   }
 }
@@ -143,7 +146,37 @@ __global__ __launch_bounds__(kBlock) void moments_kernel(StageParams sp) {
   }
   block_sum(acc, red);
   if (threadIdx.x == 0) {
-    for (int k = 0; k < 5; ++k) sp.mom_out[k * sp.Q + q] = acc[k] * sp.measure;
+    for (int k = 0; k < 5; ++k) acc[k] *= sp.measure;
+    write_moment_block(sp.mom_out, sp.Q, q, acc, sp.mass, sp.kB, sp.p_dim);
+  }
+}
+
+// Slab order of the p-cells for qcell_kernel: grouped by the signs of
+// (v1, v2) -- (<=0,<=0), (>0,<=0), (<=0,>0), (>0,>0) -- each group in natural
+// order and padded with -1 to a multiple of 64; jpack holds j1 | j2<<10 | j3<<20
+// of each entry (0 for padding).  One thread; run once per problem.
+__global__ void slab_perm_kernel(int* perm, int* jpack, int n_p1, int n_p2, int n_p3, double p1s,
+                                 double dp1, double p2s, double dp2) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int P = n_p1 * n_p2 * n_p3;
+  int k = 0;
+  for (int grp = 0; grp < 4; ++grp) {
+    const bool pos1 = grp & 1, pos2 = grp >> 1;
+    const int start = k;
+    for (int p = 0; p < P; ++p) {
+      const int j1 = p % n_p1, j2 = (p / n_p1) % n_p2, j3 = p / (n_p1 * n_p2);
+      if ((centre(p1s, j1, dp1) > 0.0) == pos1 && (centre(p2s, j2, dp2) > 0.0) == pos2) {
+        perm[k] = p;
+        jpack[k] = j1 | (j2 << 10) | (j3 << 20);
+        ++k;
+      }
+    }
+    if (k > start)
+      while (k % 64 != 0) {
+        perm[k] = -1;
+        jpack[k] = 0;
+        ++k;
+      }
   }
 }
 
@@ -262,6 +295,11 @@ static int check_grid(const boltx_grid* g) {
                 g->n_q1, g->n_q2, g->n_p1, g->n_p2, g->n_p3);
   if (g->n_species != 1)
     return fail(BOLTX_EUNSUPPORTED, "n_species = %d: only one species is supported", g->n_species);
+  if (g->layout != BOLTX_LAYOUT_REFERENCE && g->layout != BOLTX_LAYOUT_Q1_LINES)
+    return fail(BOLTX_EINVAL, "layout = %d", g->layout);
+  if (g->layout == BOLTX_LAYOUT_Q1_LINES && g->n_q1 > 256)
+    return fail(BOLTX_EUNSUPPORTED, "the q1-line layout holds at most 256 q1 cells (got %d)",
+                g->n_q1);
   const long P = (long)g->n_p1 * g->n_p2 * g->n_p3;
   const long Q = (long)g->n_q1 * g->n_q2;
   if (P * Q > (1L << 40)) return fail(BOLTX_EINVAL, "grid too large");
@@ -332,6 +370,150 @@ static int launch_stage_rq(const StageParams& sp, int rq, int rp, bool padv, hip
   return fail(BOLTX_EINVAL, "recon_q = %d", rq);
 }
 
+// Slab count of the sign-grouped p order (must agree with slab_perm_kernel).
+static int qc_nslab(const boltx_grid* g) {
+  int n1neg = 0, n2neg = 0;
+  for (int i = 0; i < g->n_p1; ++i) n1neg += centre(g->p1_start, i, g->dp1) > 0.0 ? 0 : 1;
+  for (int i = 0; i < g->n_p2; ++i) n2neg += centre(g->p2_start, i, g->dp2) > 0.0 ? 0 : 1;
+  const long n1[2] = {n1neg, g->n_p1 - n1neg}, n2[2] = {n2neg, g->n_p2 - n2neg};
+  long slabs = 0;
+  for (int grp = 0; grp < 4; ++grp) slabs += (n1[grp & 1] * n2[grp >> 1] * g->n_p3 + 63) / 64;
+  return (int)slabs;
+}
+
+// The slab-ordered kernel needs per-axis p sizes below 1024 (packed indices).
+static bool qc_applies(const boltx_grid* g) {
+  return g->n_p1 < 1024 && g->n_p2 < 1024 && g->n_p3 < 1024;
+}
+
+static size_t qc_workspace_bytes(int nslab) { return 2 * (size_t)nslab * 64 * sizeof(int); }
+
+template <typename T, int RQ, int RP, bool PADV>
+static int launch_qc_t(const QCParams& qp, hipStream_t s) {
+  const StageParams& sp = qp.sp;
+  size_t lds = sizeof(T) * (size_t)(sp.n_p1 + sp.n_p2 + sp.n_p3);
+  if (PADV) lds += sizeof(T) * (size_t)sp.P;
+  if (lds > 160 * 1024) return fail(BOLTX_EUNSUPPORTED, "p-block of %zu bytes exceeds LDS", lds);
+  hipLaunchKernelGGL((qcell_kernel<T, RQ, RP, PADV>), dim3(sp.Q), dim3(256), lds, s, qp);
+  return launch_status("qcell_kernel");
+}
+
+template <typename T, int RQ>
+static int launch_qc_rp(const QCParams& qp, int rp, bool padv, hipStream_t s) {
+  if (!padv) return launch_qc_t<T, RQ, RC_PC, false>(qp, s);
+  switch (rp) {
+    case RC_PC: return launch_qc_t<T, RQ, RC_PC, true>(qp, s);
+    case RC_MINMOD: return launch_qc_t<T, RQ, RC_MINMOD, true>(qp, s);
+    case RC_PPM: return launch_qc_t<T, RQ, RC_PPM, true>(qp, s);
+    default: return launch_qc_t<T, RQ, RC_WENO5, true>(qp, s);
+  }
+}
+
+template <typename T>
+static int launch_qc(const QCParams& qp, int rq, int rp, bool padv, hipStream_t s) {
+  switch (rq) {
+    case RC_PC: return launch_qc_rp<T, RC_PC>(qp, rp, padv, s);
+    case RC_MINMOD: return launch_qc_rp<T, RC_MINMOD>(qp, rp, padv, s);
+    case RC_PPM: return launch_qc_rp<T, RC_PPM>(qp, rp, padv, s);
+    default: return launch_qc_rp<T, RC_WENO5>(qp, rp, padv, s);
+  }
+}
+
+// ---- q1-line layout --------------------------------------------------------
+constexpr int ML_CHUNKS = 4;
+
+static int ql_segment(int n2) {
+  for (int L = 32; L > 1; L /= 2)
+    if (n2 % L == 0) return L;
+  return 1;
+}
+
+static size_t ql_table_bytes(const boltx_grid* g, int precision) {
+  const size_t nj = (size_t)g->n_p1 + g->n_p2 + g->n_p3;
+  const size_t Q = (size_t)g->n_q1 * g->n_q2;
+  return (nj * Q * (precision == BOLTX_FP64 ? 8 : 4) + 255) / 256 * 256;
+}
+
+static size_t ql_workspace_bytes(const boltx_grid* g, int precision) {
+  return ql_table_bytes(g, precision) + (size_t)ML_CHUNKS * 5 * g->n_q1 * g->n_q2 * sizeof(double);
+}
+
+template <typename T>
+static int ql_moments(const boltx_grid* g, const StageParams& sp, const void* f, double* mom,
+                      double* part, hipStream_t s) {
+  const int nblk = (g->n_q1 + 63) / 64;
+  hipLaunchKernelGGL(mline_kernel<T>, dim3(nblk * g->n_q2 * ML_CHUNKS), dim3(256), 0, s,
+                     (const T*)f, g->n_q1, g->n_q2, sp.P, ML_CHUNKS, g->n_p1, g->n_p2,
+                     g->p1_start, g->dp1, g->p2_start, g->dp2, g->p3_start, g->dp3, part);
+  int rc = launch_status("mline_kernel");
+  if (rc) return rc;
+  hipLaunchKernelGGL(ml_finalize_kernel, dim3((sp.Q + 255) / 256), dim3(256), 0, s, part,
+                     ML_CHUNKS, sp.Q, sp.measure, mom, sp.mass, sp.kB, sp.p_dim);
+  return launch_status("ml_finalize_kernel");
+}
+
+template <typename T, int R>
+static void launch_ql_nr(const QLParams& q, int nb, hipStream_t s) {
+  if (q.n1 <= 64) hipLaunchKernelGGL((qline_kernel<T, R, 1>), dim3(nb), dim3(256), 0, s, q);
+  else if (q.n1 <= 128) hipLaunchKernelGGL((qline_kernel<T, R, 2>), dim3(nb), dim3(256), 0, s, q);
+  else hipLaunchKernelGGL((qline_kernel<T, R, 4>), dim3(nb), dim3(256), 0, s, q);
+}
+
+template <typename T>
+static int ql_stage(const boltx_grid* g, const boltx_physics* ph, const boltx_stage* st,
+                    const StageParams& sp, hipStream_t s) {
+  char* ws = (char*)st->workspace;
+  T* tab = (T*)ws;
+  double* part = (double*)(ws + ql_table_bytes(g, ph->precision));
+  int rc;
+  if (sp.source) {
+    const long n = (long)(g->n_p1 + g->n_p2 + g->n_p3) * sp.Q;
+    hipLaunchKernelGGL(bgk_table_kernel<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       st->mom_in, sp.Q, g->n_p1, g->n_p2, g->n_p3, g->p1_start, g->dp1,
+                       g->p2_start, g->dp2, g->p3_start, g->dp3, sp.mass, sp.kB, tab);
+    if ((rc = launch_status("bgk_table_kernel"))) return rc;
+  }
+  QLParams q;
+  memset(&q, 0, sizeof(q));
+  q.n1 = g->n_q1;
+  q.n2 = g->n_q2;
+  q.P = sp.P;
+  q.L = ql_segment(g->n_q2);
+  q.nseg = g->n_q2 / q.L;
+  q.n_p1 = g->n_p1;
+  q.n_p2 = g->n_p2;
+  q.n_p3 = g->n_p3;
+  q.u = st->u;
+  q.w = st->w;
+  q.un = st->un;
+  q.out = st->out;
+  q.a = st->a;
+  q.b = st->b;
+  q.cdt = st->cdt;
+  q.inv_dq1 = sp.inv_dq1;
+  q.inv_dq2 = sp.inv_dq2;
+  q.inv_tau = sp.inv_tau;
+  q.source = sp.source;
+  q.p_dim = sp.p_dim;
+  q.p1s = g->p1_start;
+  q.dp1 = g->dp1;
+  q.p2s = g->p2_start;
+  q.dp2 = g->dp2;
+  q.p3s = g->p3_start;
+  q.dp3 = g->dp3;
+  q.tab = tab;
+  const int nb = (sp.P + 3) / 4 * q.nseg;
+  switch (ph->recon_q) {
+    case RC_PC: launch_ql_nr<T, RC_PC>(q, nb, s); break;
+    case RC_MINMOD: launch_ql_nr<T, RC_MINMOD>(q, nb, s); break;
+    case RC_PPM: launch_ql_nr<T, RC_PPM>(q, nb, s); break;
+    default: launch_ql_nr<T, RC_WENO5>(q, nb, s);
+  }
+  if ((rc = launch_status("qline_kernel"))) return rc;
+  if (st->mom_out) return ql_moments<T>(g, sp, st->out, st->mom_out, part, s);
+  return BOLTX_OK;
+}
+
 static int check_physics(const boltx_physics* ph) {
   if (!ph) return fail(BOLTX_EINVAL, "physics is NULL");
   if (ph->recon_q < 0 || ph->recon_q > 3 || ph->recon_p < 0 || ph->recon_p > 3)
@@ -391,27 +573,73 @@ extern "C" int boltx_fvm_stage(const boltx_grid* grid, const boltx_physics* phys
   sp.fp3 = st->fields_c_p3;
   hipStream_t s = (hipStream_t)stream;
   const bool padv = phys->p_advection != 0;
+  if (grid->layout == BOLTX_LAYOUT_Q1_LINES) {
+    if (padv) return fail(BOLTX_EUNSUPPORTED, "p-advection needs the reference layout");
+    if (!st->workspace) return fail(BOLTX_EINVAL, "the q1-line layout needs the workspace");
+    if (phys->precision == BOLTX_FP64) return ql_stage<double>(grid, phys, st, sp, s);
+    return ql_stage<float>(grid, phys, st, sp, s);
+  }
+  if (st->workspace && qc_applies(grid)) {
+    QCParams qp;
+    qp.sp = sp;
+    qp.nslab = qc_nslab(grid);
+    qp.perm = (const int*)st->workspace;
+    qp.jpack = qp.perm + (size_t)qp.nslab * 64;
+    if (phys->precision == BOLTX_FP64)
+      return launch_qc<double>(qp, phys->recon_q, phys->recon_p, padv, s);
+    return launch_qc<float>(qp, phys->recon_q, phys->recon_p, padv, s);
+  }
   if (phys->precision == BOLTX_FP64)
     return launch_stage_rq<double>(sp, phys->recon_q, phys->recon_p, padv, s);
   return launch_stage_rq<float>(sp, phys->recon_q, phys->recon_p, padv, s);
 }
 
-extern "C" int boltx_moments(const boltx_grid* grid, int32_t precision, const void* f,
-                             double* mom, void* stream) {
+extern "C" int boltx_moments(const boltx_grid* grid, const boltx_physics* phys, const void* f,
+                             double* mom, void* workspace, void* stream) {
   int rc = check_grid(grid);
   if (rc) return rc;
+  if ((rc = check_physics(phys))) return rc;
   if (!f || !mom) return fail(BOLTX_EINVAL, "moments pointers missing");
   StageParams sp = base_params(grid);
   sp.u = f;
   sp.mom_out = mom;
+  sp.mass = phys->mass;
+  sp.kB = phys->boltzmann_constant;
+  sp.p_dim = phys->p_dim;
   hipStream_t s = (hipStream_t)stream;
-  if (precision == BOLTX_FP64)
+  if (grid->layout == BOLTX_LAYOUT_Q1_LINES) {
+    if (!workspace) return fail(BOLTX_EINVAL, "the q1-line layout needs the workspace");
+    double* part = (double*)((char*)workspace + ql_table_bytes(grid, phys->precision));
+    if (phys->precision == BOLTX_FP64) return ql_moments<double>(grid, sp, f, mom, part, s);
+    return ql_moments<float>(grid, sp, f, mom, part, s);
+  }
+  if (phys->precision == BOLTX_FP64)
     hipLaunchKernelGGL(moments_kernel<double>, dim3(sp.Q), dim3(kBlock), 0, s, sp);
-  else if (precision == BOLTX_FP32_BULK)
-    hipLaunchKernelGGL(moments_kernel<float>, dim3(sp.Q), dim3(kBlock), 0, s, sp);
   else
-    return fail(BOLTX_EINVAL, "precision = %d", precision);
+    hipLaunchKernelGGL(moments_kernel<float>, dim3(sp.Q), dim3(kBlock), 0, s, sp);
   return launch_status("moments_kernel");
+}
+
+extern "C" size_t boltx_workspace_size(const boltx_grid* grid, const boltx_physics* phys) {
+  if (check_grid(grid) || check_physics(phys)) return 0;
+  if (grid->layout == BOLTX_LAYOUT_Q1_LINES) return ql_workspace_bytes(grid, phys->precision);
+  if (!qc_applies(grid)) return 0;
+  return qc_workspace_bytes(qc_nslab(grid));
+}
+
+extern "C" int boltx_workspace_init(const boltx_grid* grid, const boltx_physics* phys,
+                                    void* workspace, void* stream) {
+  int rc = check_grid(grid);
+  if (rc) return rc;
+  if ((rc = check_physics(phys))) return rc;
+  if (grid->layout == BOLTX_LAYOUT_Q1_LINES || !qc_applies(grid)) return BOLTX_OK;
+  if (!workspace) return fail(BOLTX_EINVAL, "workspace is NULL");
+  const int nslab = qc_nslab(grid);
+  int* perm = (int*)workspace;
+  hipLaunchKernelGGL(slab_perm_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, perm,
+                     perm + (size_t)nslab * 64, grid->n_p1, grid->n_p2, grid->n_p3,
+                     grid->p1_start, grid->dp1, grid->p2_start, grid->dp2);
+  return launch_status("slab_perm_kernel");
 }
 
 template <typename T>
@@ -432,6 +660,8 @@ extern "C" int boltx_currents(const boltx_grid* grid, const boltx_physics* phys,
   if (rc) return rc;
   if ((rc = check_physics(phys))) return rc;
   if (!u || !J) return fail(BOLTX_EINVAL, "currents pointers missing");
+  if (grid->layout != BOLTX_LAYOUT_REFERENCE)
+    return fail(BOLTX_EUNSUPPORTED, "currents need the reference layout");
   StageParams sp = base_params(grid);
   sp.u = u;
   sp.charge = phys->charge;

@@ -242,6 +242,27 @@ __host__ __device__ inline double maxwell_factor(double v, double u, double T, d
   return exp(-mass * (d * d) / (2.0 * kB * T));
 }
 
+// A moment block holds, per q-cell, the five built-in moments (density,
+// mom_v1_bulk, mom_v2_bulk, mom_v3_bulk, energy; rows 0-4) followed by the
+// BGK parameters derived from them (rows 5-11): prefactor n (m/2 pi k T)^(p_dim/2),
+// u1, u2, u3, T, n, m/(2 k T).
+constexpr int MOM_ROWS = 12;
+constexpr int MOM_BGK = 5;
+
+__host__ __device__ inline void write_moment_block(double* mom, int Q, int qc, const double m[5],
+                                                   double mass, double kB, int p_dim) {
+  for (int k = 0; k < 5; ++k) mom[k * Q + qc] = m[k];
+  const BgkCell c = bgk_cell(m[0], m[1], m[2], m[3], m[4], mass, p_dim);
+  double* b = mom + MOM_BGK * Q + qc;
+  b[0] = maxwell_prefactor(c, mass, kB, p_dim);
+  b[Q] = c.u1;
+  b[2 * Q] = c.u2;
+  b[3 * Q] = c.u3;
+  b[4 * Q] = c.T;
+  b[5 * Q] = c.n;
+  b[6 * Q] = mass / (2.0 * kB * c.T);
+}
+
 // Cell-centre coordinate, calculate_p.py:48-50: start + (0.5 + i) * d.
 __host__ __device__ inline double centre(double start, int i, double d) {
   return start + (0.5 + double(i)) * d;

@@ -17,7 +17,8 @@ path: construction fails if the HIP library or the GPU is missing.
 Additions to the reference's parameters (all optional):
   ``riemann_solver_in_q/p``: 'upwind-flux' (reference) or 'lax-friedrichs';
   ``time_stepper_fvm``: 'RK2' (reference, fvm_operator.py:6-35) or 'SSP-RK3';
-  ``precision``: 'fp64' (default) or 'fp32-bulk'.
+  ``precision``: 'fp64' (default) or 'fp32-bulk';
+  ``memory_layout``: 'auto' (default), 'reference' or 'q1-lines'.
 """
 import ctypes
 import math
@@ -86,7 +87,7 @@ class nonlinear_solver(object):
         self._configure(ps, p)
         self._build_grids(ps)
         self._grid = _lib.Grid(self.N_q1, self.N_q2, self.N_p1, self.N_p2, self.N_p3, 1,
-                               float(self.dq1), float(self.dq2), self._dp[0], self._dp[1],
+                               self._layout, float(self.dq1), float(self.dq2), self._dp[0], self._dp[1],
                                self._dp[2], float(ps.p1_start[0]), float(ps.p2_start[0]),
                                float(ps.p3_start[0]))
         self._phys = _lib.Physics(self._recon_q, self._recon_p, self._riemann_q,
@@ -98,7 +99,14 @@ class nonlinear_solver(object):
         Q, P = self.N_q1 * self.N_q2, self.N_p1 * self.N_p2 * self.N_p3
         self._Q, self._P = Q, P
         self._bufs = {}
-        self._mom = torch.zeros(5, Q, dtype=F64, device=self.device)
+        self._mom = torch.zeros(_lib.MOM_ROWS, Q, dtype=F64, device=self.device)
+        nws = int(self._L.boltx_workspace_size(ctypes.byref(self._grid), ctypes.byref(self._phys)))
+        self._ws = None
+        if nws > 0:
+            self._ws = torch.empty(nws, dtype=torch.uint8, device=self.device)
+            _lib.check(self._L.boltx_workspace_init(ctypes.byref(self._grid),
+                                                    ctypes.byref(self._phys),
+                                                    _lib.ptr(self._ws), _lib.stream_handle()))
         self._flag = torch.zeros(1, dtype=torch.int32, device=self.device)
         self.time_elapsed = 0
         self.dt = None
@@ -128,6 +136,18 @@ class nonlinear_solver(object):
         self._precision = _lib.PRECISION[prec]
         self._dtype = torch.float64 if prec == 'fp64' else torch.float32
         self._stepper = getattr(p, 'time_stepper_fvm', 'RK2')
+        # memory layout of f: the q1-line layout [q2][p][q1] for fields-off
+        # problems whose q1 extent fills whole 64-wide waves, else the
+        # reference byte order [q2][q1][p]
+        lay = getattr(p, 'memory_layout', 'auto')
+        fields = bool(getattr(p, 'fields_enabled', False))
+        if lay == 'auto':
+            lay = 'q1-lines' if (not fields and ps.N_q1 in (64, 128, 256)) else 'reference'
+        if lay not in _lib.LAYOUT:
+            raise NotImplementedError("memory_layout must be 'auto', 'reference' or 'q1-lines'")
+        if lay == 'q1-lines' and (fields or ps.N_q1 > 256):
+            raise NotImplementedError('the q1-line layout needs fields off and N_q1 <= 256')
+        self._layout = _lib.LAYOUT[lay]
         if self._stepper not in ('RK2', 'SSP-RK3'):
             raise NotImplementedError('time_stepper_fvm must be RK2 or SSP-RK3')
         if ps.C_q is not _adv.C_q:
@@ -210,18 +230,21 @@ class nonlinear_solver(object):
         return a[i1][:, i2].reshape(1, 1, len(i1), len(i2))
 
     def _to_internal(self, f_ref):
-        """(P, S, Q1g, Q2g) -> contiguous [N_q2][N_q1][P] in the storage dtype."""
+        """(P, S, Q1g, Q2g) -> the contiguous storage layout ([N_q2][N_q1][P]
+        or [N_q2][P][N_q1]) in the storage dtype."""
         f_ref = torch.as_tensor(f_ref, device=self.device)
         f_ref = torch.broadcast_to(f_ref, (self._P, 1) + tuple(self.q1_center.shape[2:]))
-        inner = self._q_interior(f_ref)[:, 0]
-        return inner.permute(2, 1, 0).contiguous().to(self._dtype)
+        inner = self._q_interior(f_ref)[:, 0]  # (P, N_q1, N_q2)
+        perm = (2, 1, 0) if self._layout == 0 else (2, 0, 1)
+        return inner.permute(*perm).contiguous().to(self._dtype)
 
     def _to_reference(self, f_int):
-        """[N_q2][N_q1][P] -> (P, 1, Q1g, Q2g) with periodic ghost zones."""
+        """Storage layout -> (P, 1, Q1g, Q2g) with periodic ghost zones."""
         i1 = torch.arange(-self.N_g1, self.N_q1 + self.N_g1, device=f_int.device) % self.N_q1
         i2 = torch.arange(-self.N_g2, self.N_q2 + self.N_g2, device=f_int.device) % self.N_q2
-        padded = f_int[i2][:, i1]
-        return padded.permute(2, 1, 0).unsqueeze(1)
+        if self._layout == 0:
+            return f_int[i2][:, i1].permute(2, 1, 0).unsqueeze(1)
+        return f_int[i2][:, :, i1].permute(1, 2, 0).unsqueeze(1)
 
     @property
     def f(self):
@@ -245,13 +268,13 @@ class nonlinear_solver(object):
         self.fields_solver = _fields_solver(self) if self._fields_on else None
 
     def _update_moments(self):
-        _lib.check(self._L.boltx_moments(ctypes.byref(self._grid), self._precision,
+        _lib.check(self._L.boltx_moments(ctypes.byref(self._grid), ctypes.byref(self._phys),
                                          _lib.ptr(self._f), _lib.ptr(self._mom),
-                                         _lib.stream_handle()))
+                                         _lib.ptr(self._ws), _lib.stream_handle()))
 
     def _buf(self, name, dtype=None, shape=None):
         dtype = dtype or self._dtype
-        shape = shape or (self.N_q2, self.N_q1, self._P)
+        shape = shape or tuple(self._f.shape)
         t = self._bufs.get(name)
         if t is None or t.dtype != dtype or tuple(t.shape) != tuple(shape):
             t = torch.empty(shape, dtype=dtype, device=self.device)
@@ -274,7 +297,7 @@ class nonlinear_solver(object):
             F = [fs._take_for_C_p() for _ in range(3)]  # df_dt_fvm.py:251-267
         st = _lib.Stage(_lib.ptr(u), _lib.ptr(w), _lib.ptr(un), _lib.ptr(out),
                         float(a), float(b), float(cdt), _lib.ptr(mom_in), _lib.ptr(mom_out),
-                        _lib.ptr(F[0]), _lib.ptr(F[1]), _lib.ptr(F[2]))
+                        _lib.ptr(F[0]), _lib.ptr(F[1]), _lib.ptr(F[2]), _lib.ptr(self._ws))
         _lib.check(self._L.boltx_fvm_stage(ctypes.byref(self._grid), ctypes.byref(self._phys),
                                            ctypes.byref(st), _lib.stream_handle()))
 
@@ -285,12 +308,12 @@ class nonlinear_solver(object):
         f = self._f
         if self._stepper == 'RK2':
             # fvm_operator.py:25-33
-            u1, m1 = self._buf('u1'), self._buf('m1', F64, (5, self._Q))
+            u1, m1 = self._buf('u1'), self._buf('m1', F64, (_lib.MOM_ROWS, self._Q))
             self._stage(f, f, None, u1, 0.0, 1.0, dt / 2, self._mom, m1)
             self._stage(u1, f, None, f, 0.0, 1.0, dt, m1, self._mom)
         else:
-            u1, m1 = self._buf('u1'), self._buf('m1', F64, (5, self._Q))
-            u2, m2 = self._buf('u2'), self._buf('m2', F64, (5, self._Q))
+            u1, m1 = self._buf('u1'), self._buf('m1', F64, (_lib.MOM_ROWS, self._Q))
+            u2, m2 = self._buf('u2'), self._buf('m2', F64, (_lib.MOM_ROWS, self._Q))
             self._stage(f, f, None, u1, 0.0, 1.0, dt, self._mom, m1)
             self._stage(u1, u1, f, u2, 0.75, 0.25, dt, m1, m2)
             self._stage(u2, u2, f, f, 1 / 3, 2 / 3, dt, m2, self._mom)

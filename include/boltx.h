@@ -13,18 +13,25 @@
  *     in boltx_last_error() (thread-local).
  *
  * Layout of a distribution function: interior q-cells only (periodic q is
- * handled by modular indexing, so there are no ghost zones), C-order
- * [n_q2][n_q1][P] with P = n_p1*n_p2*n_p3 and p-index i1 + n_p1*(i2 + n_p2*i3)
- * (p1 fastest), i.e. the byte order of the reference's (P, S, Q1, Q2)
- * column-major af.Array for S = 1 without its ghost rows.  FP64 storage is
- * double, FP32-bulk storage is float.
- * Moments are FP64 arrays [5][Q] (Q = n_q1*n_q2): density, mom_v1_bulk,
- * mom_v2_bulk, mom_v3_bulk, energy, each already multiplied by dp3*dp2*dp1.
+ * handled by modular indexing, so there are no ghost zones), P = n_p1*n_p2*n_p3
+ * with p-index i1 + n_p1*(i2 + n_p2*i3) (p1 fastest), and either
+ *   BOLTX_LAYOUT_REFERENCE: C-order [n_q2][n_q1][P], the byte order of the
+ *     reference's (P, S, Q1, Q2) column-major af.Array for S = 1 without its
+ *     ghost rows; or
+ *   BOLTX_LAYOUT_Q1_LINES: C-order [n_q2][P][n_q1] (q1 fastest, n_q1 <= 256),
+ *     used for fields-off problems: every q1 line of one p-cell is contiguous.
+ * FP64 storage is double, FP32-bulk storage is float.
+ * A moment block is an FP64 array [BOLTX_MOM_ROWS][Q] (Q = n_q1*n_q2): rows
+ * 0-4 are density, mom_v1_bulk, mom_v2_bulk, mom_v3_bulk and energy, each
+ * already multiplied by dp3*dp2*dp1; rows 5-11 are the BGK parameters derived
+ * from them (n (m/2 pi k T)^(p_dim/2), u1, u2, u3, T, n, m/(2kT)).  Every entry
+ * point that writes a moment block writes all of it.
  * Cell-centred fields are FP64 arrays [6][Q]: E1, E2, E3, B1, B2, B3.
  */
 #ifndef BOLTX_H
 #define BOLTX_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -32,6 +39,7 @@ extern "C" {
 #endif
 
 #define BOLTX_VERSION 1
+#define BOLTX_MOM_ROWS 12
 
 enum boltx_recon {
   BOLTX_RECON_PIECEWISE_CONSTANT = 0,
@@ -41,6 +49,7 @@ enum boltx_recon {
 };
 enum boltx_riemann { BOLTX_RIEMANN_UPWIND = 0, BOLTX_RIEMANN_LLF = 1 };
 enum boltx_precision { BOLTX_FP64 = 0, BOLTX_FP32_BULK = 1 };
+enum boltx_layout { BOLTX_LAYOUT_REFERENCE = 0, BOLTX_LAYOUT_Q1_LINES = 1 };
 enum boltx_status {
   BOLTX_OK = 0,
   BOLTX_EINVAL = -1,
@@ -52,6 +61,7 @@ typedef struct boltx_grid {
   int32_t n_q1, n_q2;       /* interior q-cells, periodic */
   int32_t n_p1, n_p2, n_p3; /* p-cells per axis */
   int32_t n_species;        /* 1 */
+  int32_t layout;           /* enum boltx_layout of every f-sized array */
   double dq1, dq2;
   double dp1, dp2, dp3;
   double p1_start, p2_start, p3_start;
@@ -74,11 +84,13 @@ typedef struct boltx_stage {
   const void *un;  /* step-start state, read pointwise when a != 0 */
   void *out;       /* stage output; must not alias u (may alias w or un) */
   double a, b, cdt; /* out = a*un + b*(w + cdt*L(u)) */
-  const double *mom_in;  /* [5][Q] moments of u (needed when the source is on) */
-  double *mom_out;       /* [5][Q] moments of out, or NULL */
+  const double *mom_in;  /* moment block of u (needed when the source is on) */
+  double *mom_out;       /* moment block of out, or NULL */
   const double *fields_c_p1; /* [6][Q] fields seen by C_p1 (p_advection) */
   const double *fields_c_p2; /* [6][Q] fields seen by C_p2 */
   const double *fields_c_p3; /* [6][Q] fields seen by C_p3 */
+  void *workspace;  /* boltx_workspace_size() bytes set up by boltx_workspace_init(),
+                       or NULL; stages sharing a workspace must be stream-ordered */
 } boltx_stage;
 
 /* One Runge-Kutta stage of the FVM step: out = a*un + b*(w + cdt*L(u)) where
@@ -90,11 +102,21 @@ typedef struct boltx_stage {
 int boltx_fvm_stage(const boltx_grid *grid, const boltx_physics *phys,
                     const boltx_stage *stage, void *stream);
 
-/* The five built-in moments of f, [5][Q] FP64.  Replaces compute_moments
+/* The moment block of f.  Replaces compute_moments
  * (bolt/lib/nonlinear/compute_moments.py:7-59) for density, mom_v1_bulk,
- * mom_v2_bulk, mom_v3_bulk and energy (moments.py:5-22). */
-int boltx_moments(const boltx_grid *grid, int32_t precision, const void *f,
-                  double *mom, void *stream);
+ * mom_v2_bulk, mom_v3_bulk and energy (moments.py:5-22), plus the BGK
+ * parameters (collision_operator.py:102-116). */
+int boltx_moments(const boltx_grid *grid, const boltx_physics *phys, const void *f,
+                  double *mom, void *workspace, void *stream);
+
+/* Bytes of device workspace boltx_fvm_stage and boltx_moments use for this
+ * problem (required in the q1-line layout, optional in the reference layout,
+ * where it holds the sign-grouped p order), and its one-time set-up.  The
+ * reference has no counterpart: ArrayFire allocates its temporaries itself;
+ * here the caller owns all device memory. */
+size_t boltx_workspace_size(const boltx_grid *grid, const boltx_physics *phys);
+int boltx_workspace_init(const boltx_grid *grid, const boltx_physics *phys,
+                         void *workspace, void *stream);
 
 /* Currents J1, J2, J3 ([3][Q] FP64) from the upwind face states of u:
  * J1 = charge * mom_v1_bulk(f_left), J2 = charge * mom_v2_bulk(f_bot),

@@ -31,7 +31,7 @@ def maxwellian_case(*, N_q1, N_q2, N_p1, N_p2=1, N_p3=1, N_ghost=3, p_max=10.0,
                     recon_q="weno5", recon_p="weno5", riemann="upwind-flux",
                     stepper="SSP-RK3", tau=0.01, source=True, fields=False,
                     B0=0.5, charge=-1.0, mass=1.0, precision="fp64",
-                    q2_bc="periodic", p3_max=None):
+                    q2_bc="periodic", p3_max=None, memory_layout="auto"):
     """Maxwellian (T = 1, u = 0) with density n = 1 + sum A_k cos(2 pi (kx q1 +
     ky q2) + phi_k) on [0,1)^2, p in [-p_max, p_max] per active axis (singleton
     axes span [-0.5, 0.5] as in the reference examples)."""
@@ -55,7 +55,7 @@ def maxwellian_case(*, N_q1, N_q2, N_p1, N_p2=1, N_p3=1, N_ghost=3, p_max=10.0,
         num_devices=1, mass=[mass], boltzmann_constant=1, charge=[charge],
         fields_enabled=fields, source_enabled=source, instantaneous_collisions=False,
         hybrid_model_enabled=False, eps=1, mu=1, tau=_const_tau(tau),
-        time_stepper_fvm=stepper, precision=precision, B0=B0,
+        time_stepper_fvm=stepper, precision=precision, B0=B0, memory_layout=memory_layout,
         modes=tuple(modes), amps=tuple(amps), phases=tuple(phases))
 
     def initialize_f(q1, q2, p1, p2, p3, params):

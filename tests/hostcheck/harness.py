@@ -33,7 +33,7 @@ def to_internal(o, x):
 
 def grid_physics(o, precision=0, source=None, padv=None):
     p = o.params
-    g = L.Grid(o.N_q1, o.N_q2, o.N_p1, o.N_p2, o.N_p3, 1, o.dq1, o.dq2,
+    g = L.Grid(o.N_q1, o.N_q2, o.N_p1, o.N_p2, o.N_p3, 1, 0, o.dq1, o.dq2,
                float(o.dp1), float(o.dp2), float(o.dp3),
                float(o.physical_system.p1_start[0]), float(o.physical_system.p2_start[0]),
                float(o.physical_system.p3_start[0]))
@@ -74,6 +74,6 @@ def stage(o, u, w, un, a, b, cdt, precision=0, contract="fast-honor-pragmas", fi
     P = lambda x: None if x is None else x.ctypes.data_as(ctypes.c_void_p)
     F = [None, None, None] if fields is None else [np.ascontiguousarray(x) for x in fields]
     st = L.Stage(P(u), P(w), P(un), P(out), a, b, cdt, P(mom_in), P(mom_out),
-                 P(F[0]), P(F[1]), P(F[2]))
+                 P(F[0]), P(F[1]), P(F[2]), None)
     lib(contract).host_fvm_stage(ctypes.byref(g), ctypes.byref(ph), ctypes.byref(st))
     return out, mom_out

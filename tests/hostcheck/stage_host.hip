@@ -99,3 +99,39 @@ extern "C" int host_fvm_stage(const boltx_grid* g, const boltx_physics* ph, cons
   else run_rq<float>(sp, ph->recon_q, ph->recon_p, ph->p_advection != 0);
   return 0;
 }
+
+#include "../../bolt_amd/csrc/qcell.h"
+
+// Flux differences of n seven-point windows two ways: the slab kernel's
+// upwind evaluation (shared indicators, one division per FP64 face) and the
+// reference-form flux_diff.
+template <typename T, int R>
+static void flux_pair(const double* a7, const double* C, double* fast, double* ref, int n, int rs) {
+  for (int i = 0; i < n; ++i) {
+    T a[7];
+    for (int k = 0; k < 7; ++k) a[k] = T(a7[7 * i + k]);
+    const T c = T(C[i]);
+    fast[i] = double(upwind_flux_diff<R, T>(a, c, c > T(0)));
+    ref[i] = double(flux_diff<R, T>(a, c, rs, (T*)nullptr));
+  }
+}
+
+extern "C" int host_flux_pair(int recon, int precision, int riemann, const double* a7,
+                              const double* C, double* fast, double* ref, int n) {
+  if (precision == 0) {
+    switch (recon) {
+      case 0: flux_pair<double, RC_PC>(a7, C, fast, ref, n, riemann); break;
+      case 1: flux_pair<double, RC_MINMOD>(a7, C, fast, ref, n, riemann); break;
+      case 2: flux_pair<double, RC_PPM>(a7, C, fast, ref, n, riemann); break;
+      default: flux_pair<double, RC_WENO5>(a7, C, fast, ref, n, riemann);
+    }
+  } else {
+    switch (recon) {
+      case 0: flux_pair<float, RC_PC>(a7, C, fast, ref, n, riemann); break;
+      case 1: flux_pair<float, RC_MINMOD>(a7, C, fast, ref, n, riemann); break;
+      case 2: flux_pair<float, RC_PPM>(a7, C, fast, ref, n, riemann); break;
+      default: flux_pair<float, RC_WENO5>(a7, C, fast, ref, n, riemann);
+    }
+  }
+  return 0;
+}

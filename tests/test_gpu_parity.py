@@ -96,7 +96,8 @@ def test_2d2v_weno5_fine_q(gpu):
     out = torch.empty_like(nls._f)
     m = torch.empty_like(nls._mom)
     nls._stage(nls._f, nls._f, None, out, 0.0, 1.0, c.dt, nls._mom, m)
-    got = out.permute(2, 1, 0).unsqueeze(1).cpu().numpy()
+    g1, g2 = nls.N_g1, nls.N_g2
+    got = nls._to_reference(out)[:, :, g1:g1 + nls.N_q1, g2:g2 + nls.N_q2].cpu().numpy()
     err = np.abs(got - ref).max() / np.abs(ref).max()
     print(f"one RHS evaluation at 128^2: rel Linf {err:.3e}", flush=True)
     assert err <= 1e-12
@@ -170,3 +171,39 @@ def test_full_size_2d2v_conservation_and_fp32(gpu):
     l2 = (torch.linalg.vector_norm(fa - fb) / torch.linalg.vector_norm(fb)).item()
     print(f"full size FP32-bulk vs FP64 after 5 steps: rel L2 {l2:.3e}", flush=True)
     assert l2 <= 1e-5
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32-bulk"])
+@pytest.mark.parametrize("rq", ["weno5", "minmod"])
+def test_layouts_agree_full_size(gpu, precision, rq):
+    """The q1-line layout path (q1-line kernel + streaming moments) and the
+    reference-layout path (one workgroup per q-cell, with and without the
+    sign-grouped p order) compute the same SSP-RK3 stage on the full 2D2V grid,
+    including the moment block of the output."""
+    outs, moms = [], []
+    for layout, use_ws in (("q1-lines", True), ("reference", True), ("reference", False)):
+        c = case_2d2v(seed=0, precision=precision, recon_q=rq, memory_layout=layout)
+        nls = hip_solver(c)
+        if not use_ws:
+            nls._ws = None
+        f = nls._f
+        out = torch.empty_like(f)
+        m = torch.empty_like(nls._mom)
+        nls._stage(f, f, f, out, 0.25, 0.75, c.dt, nls._mom, m)
+        g1, g2 = nls.N_g1, nls.N_g2
+        outs.append(nls._to_reference(out)[:, :, g1:g1 + nls.N_q1, g2:g2 + nls.N_q2].double())
+        moms.append(m)
+        del nls
+    torch.cuda.synchronize()
+    ref = outs[2].abs().max().item()
+    n = moms[2][0].abs().max()
+    vmax = 10.0
+    scale = torch.stack([n, n * vmax, n * vmax, n * vmax, moms[2][4].abs().max()])
+    tol = 1e-14 if precision == "fp64" else 1e-6
+    for k in (0, 1):
+        d = (outs[k] - outs[2]).abs().max().item()
+        dm = ((moms[k][:5] - moms[2][:5]).abs().amax(dim=1) / scale).max().item()
+        print(f"path {k} vs generic ({precision}, {rq}): f rel Linf {d / ref:.3e}, "
+              f"moments {dm:.3e}", flush=True)
+        assert d <= tol * ref
+        assert dm <= (1e-13 if precision == "fp64" else 1e-6)

@@ -28,11 +28,12 @@ def test_invalid_arguments_are_rejected():
     rc = lib.boltx_fvm_stage(None, None, None, None)
     assert rc == -1
     assert b"grid" in lib.boltx_last_error()
-    g = _lib.Grid(4, 4, 8, 8, 1, 2, 0.25, 0.25, 1.0, 1.0, 1.0, -4.0, -4.0, -0.5)
-    rc = lib.boltx_moments(ctypes.byref(g), 0, ctypes.c_void_p(1), ctypes.c_void_p(1), None)
+    g = _lib.Grid(4, 4, 8, 8, 1, 2, 0, 0.25, 0.25, 1.0, 1.0, 1.0, -4.0, -4.0, -0.5)
+    ph = _lib.Physics(3, 3, 0, 0, 1, 0, 2, 7, 1.0, 1.0, 1.0, 0.01)
+    rc = lib.boltx_moments(ctypes.byref(g), ctypes.byref(ph), ctypes.c_void_p(1),
+                           ctypes.c_void_p(1), None, None)
     assert rc == -3 and b"species" in lib.boltx_last_error()
     g.n_species = 1
-    ph = _lib.Physics(3, 3, 0, 0, 1, 0, 2, 7, 1.0, 1.0, 1.0, 0.01)
     st = _lib.Stage()
     st.u = st.w = st.out = ctypes.c_void_p(8)
     rc = lib.boltx_fvm_stage(ctypes.byref(g), ctypes.byref(ph), ctypes.byref(st), None)
@@ -40,3 +41,17 @@ def test_invalid_arguments_are_rejected():
     ph.precision = 0
     rc = lib.boltx_fvm_stage(ctypes.byref(g), ctypes.byref(ph), ctypes.byref(st), None)
     assert rc == -1 and b"alias" in lib.boltx_last_error()
+    # the slab-ordered kernel needs a workspace of two int tables per slab lane
+    g2 = _lib.Grid(16, 16, 48, 48, 1, 1, 0, 1 / 16, 1 / 16, 20 / 48, 20 / 48, 1.0, -10.0, -10.0, -0.5)
+    assert lib.boltx_workspace_size(ctypes.byref(g2), ctypes.byref(ph)) == 2 * 36 * 64 * 4  # 36 slabs of 64 lanes
+    g2.n_p1 = 2000
+    assert lib.boltx_workspace_size(ctypes.byref(g2), ctypes.byref(ph)) == 0
+    # the q1-line layout needs tables (n_p1+n_p2+n_p3 rows of Q) and moment partials
+    g2.n_p1, g2.layout = 48, 1
+    tab = (48 + 48 + 1) * 256 * 8
+    assert lib.boltx_workspace_size(ctypes.byref(g2), ctypes.byref(ph)) == \
+        (tab + 255) // 256 * 256 + 4 * 5 * 256 * 8
+    g2.n_q1 = 512
+    rc = lib.boltx_moments(ctypes.byref(g2), ctypes.byref(ph), ctypes.c_void_p(1),
+                           ctypes.c_void_p(1), ctypes.c_void_p(1), None)
+    assert rc == -3 and b"256" in lib.boltx_last_error()
