@@ -420,7 +420,7 @@ static int launch_qc(const QCParams& qp, int rq, int rp, bool padv, hipStream_t 
 }
 
 // ---- q1-line layout --------------------------------------------------------
-constexpr int ML_CHUNKS = 4;
+constexpr int ML_CHUNKS = 8;
 
 static int ql_segment(int n2) {
   for (int L = 32; L > 1; L /= 2)

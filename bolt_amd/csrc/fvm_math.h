@@ -25,7 +25,8 @@ __host__ __device__ constexpr int recon_reach(int r) {
 }
 
 template <typename T> __host__ __device__ inline T amin(T a, T b) { return a < b ? a : b; }
-template <typename T> __host__ __device__ inline T aabs(T a) { return a < T(0) ? -a : a; }
+__host__ __device__ inline double aabs(double a) { return __builtin_fabs(a); }
+__host__ __device__ inline float aabs(float a) { return __builtin_fabsf(a); }
 
 // 1/x to within an ulp.  On the device: hardware reciprocal refined by Newton
 // steps (two for double); on the host: plain division.
@@ -261,6 +262,12 @@ __host__ __device__ inline void write_moment_block(double* mom, int Q, int qc, c
   b[4 * Q] = c.T;
   b[5 * Q] = c.n;
   b[6 * Q] = mass / (2.0 * kB * c.T);
+}
+
+// Periodic index: i mod n in [0, n).
+__host__ __device__ inline int qm_mod(int i, int n) {
+  i %= n;
+  return i < 0 ? i + n : i;
 }
 
 // Cell-centre coordinate, calculate_p.py:48-50: start + (0.5 + i) * d.

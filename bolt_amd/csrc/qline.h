@@ -76,49 +76,56 @@ __global__ __launch_bounds__(256) void qline_kernel(QLParams q) {
   const bool pos1 = v1 > 0.0, pos2 = v2 > 0.0;  // wave-uniform
   const T C1 = T(v1), C2 = T(v2);
   const long rowP = (long)q.P * n1;  // elements per q2 row
-  const long pofs = (long)p * n1;
   const int r0 = seg * q.L;
-  const T* u = (const T*)q.u;
-  const T* wv = (const T*)q.w;
-  const T* unv = (const T*)q.un;
-  T* outv = (T*)q.out;
-  const T* tab = (const T*)q.tab;
+  // per-wave base pointers of this p-cell; rows are added as uniform offsets
+  const T* u = (const T*)q.u + (long)p * n1;
+  const T* wv = (const T*)q.w + (long)p * n1;
+  const T* unv = (const T*)q.un + (long)p * n1;
+  T* outv = (T*)q.out + (long)p * n1;
   const long Q = (long)n1 * n2;
-  const T* t1p = tab + (long)j1 * Q;
-  const T* t2p = tab + (long)(q.n_p1 + j2) * Q;
-  const T* t3p = tab + (long)(q.n_p1 + q.n_p2 + j3) * Q;
+  const T* t1p = (const T*)q.tab + (long)j1 * Q;
+  const T* t2p = (const T*)q.tab + (long)(q.n_p1 + j2) * Q;
+  const T* t3p = (const T*)q.tab + (long)(q.n_p1 + q.n_p2 + j3) * Q;
   const bool use_un = q.a != 0.0, w_is_u = q.w == q.u;
   const T ta = T(q.a), tb = T(q.b), tcdt = T(q.cdt);
   const T tinv1 = T(q.inv_dq1), tinv2 = T(q.inv_dq2), tinvtau = T(q.inv_tau);
   const bool do_q1 = n1 > 1, do_q2 = n2 > 1;
+  const bool src = q.source != 0, src2 = src && q.p_dim >= 2, src3 = src && q.p_dim >= 3;
 
-  int i1[NR];
+  // lane cells and their periodic q1 neighbours (LDS indices), fixed for the kernel
+  int i1[NR], nm3[NR], nm2[NR], nm1[NR], np1[NR], np2[NR];
   bool live[NR];
 #pragma unroll
   for (int k = 0; k < NR; ++k) {
     i1[k] = lane + 64 * k;
     live[k] = i1[k] < n1;
     if (!live[k]) i1[k] = 0;
+    nm3[k] = qm_mod(i1[k] - 3, n1);
+    nm2[k] = qm_mod(i1[k] - 2, n1);
+    nm1[k] = qm_mod(i1[k] - 1, n1);
+    np1[k] = qm_mod(i1[k] + 1, n1);
+    np2[k] = qm_mod(i1[k] + 2, n1);
   }
-  auto nbr = [&](int i, int d) { int j = i + d; j = j < 0 ? j + n1 : j; return j >= n1 ? j - n1 : j; };
-  auto rowbase = [&](int r) -> long {
-    int rr = r % n2;
-    rr = rr < 0 ? rr + n2 : rr;
-    return (long)rr * rowP + pofs;
-  };
 
   T y[NR][7], bq[NR][4], s2m[NR], wc[NR], unc[NR], tc1[NR], tc2[NR], tc3[NR];
+  {
+    int rr = qm_mod(r0 - 3, n2);
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+      const T* row = u + rr * rowP;
+#pragma unroll
+      for (int k = 0; k < NR; ++k) y[k][j] = row[i1[k]];
+      rr = rr + 1 == n2 ? 0 : rr + 1;
+    }
+  }
+  const int rs0 = qm_mod(r0, n2);
 #pragma unroll
   for (int k = 0; k < NR; ++k) {
-#pragma unroll
-    for (int j = 0; j < 7; ++j) y[k][j] = u[rowbase(r0 - 3 + j) + i1[k]];
-    const long rb = rowbase(r0);
-    const long qc = (long)(rb / rowP) * n1 + i1[k];
-    wc[k] = w_is_u ? T(0) : wv[rb + i1[k]];
-    unc[k] = use_un ? unv[rb + i1[k]] : T(0);
-    tc1[k] = q.source ? t1p[qc] : T(0);
-    tc2[k] = (q.source && q.p_dim >= 2) ? t2p[qc] : T(0);
-    tc3[k] = (q.source && q.p_dim >= 3) ? t3p[qc] : T(0);
+    wc[k] = w_is_u ? T(0) : wv[rs0 * rowP + i1[k]];
+    unc[k] = use_un ? unv[rs0 * rowP + i1[k]] : T(0);
+    tc1[k] = src ? t1p[(long)rs0 * n1 + i1[k]] : T(0);
+    tc2[k] = src2 ? t2p[(long)rs0 * n1 + i1[k]] : T(0);
+    tc3[k] = src3 ? t3p[(long)rs0 * n1 + i1[k]] : T(0);
     s2m[k] = T(0);
 #pragma unroll
     for (int j = 0; j < 4; ++j) bq[k][j] = T(0);
@@ -132,21 +139,27 @@ __global__ __launch_bounds__(256) void qline_kernel(QLParams q) {
                                    bq[k][0], bq[k][1], bq[k][2], pos2);
     }
   }
+  // periodic row indices of r (current), r+1 and r+4
+  int rcur = rs0;
+  int rnx1 = rcur + 1 == n2 ? 0 : rcur + 1;
+  int rnx4 = qm_mod(r0 + 4, n2);
 
-  for (int r = r0; r < r0 + q.L; ++r) {
+  for (int it = 0; it < q.L; ++it) {
     // prefetch for the next row
     T yn[NR], wn[NR], unn[NR], tn1[NR], tn2[NR], tn3[NR];
     {
-      const long rb4 = rowbase(r + 4), rb1 = rowbase(r + 1);
-      const long q1base = (long)(rb1 / rowP) * n1;
+      const T* u4 = u + rnx4 * rowP;
+      const T* w1 = wv + rnx1 * rowP;
+      const T* un1 = unv + rnx1 * rowP;
+      const long tq = (long)rnx1 * n1;
 #pragma unroll
       for (int k = 0; k < NR; ++k) {
-        yn[k] = u[rb4 + i1[k]];
-        wn[k] = w_is_u ? T(0) : wv[rb1 + i1[k]];
-        unn[k] = use_un ? unv[rb1 + i1[k]] : T(0);
-        tn1[k] = q.source ? t1p[q1base + i1[k]] : T(0);
-        tn2[k] = (q.source && q.p_dim >= 2) ? t2p[q1base + i1[k]] : T(0);
-        tn3[k] = (q.source && q.p_dim >= 3) ? t3p[q1base + i1[k]] : T(0);
+        yn[k] = u4[i1[k]];
+        wn[k] = w_is_u ? T(0) : w1[i1[k]];
+        unn[k] = use_un ? un1[i1[k]] : T(0);
+        tn1[k] = src ? t1p[tq + i1[k]] : T(0);
+        tn2[k] = src2 ? t2p[tq + i1[k]] : T(0);
+        tn3[k] = src3 ? t3p[tq + i1[k]] : T(0);
       }
     }
     T d[NR];
@@ -172,11 +185,11 @@ __global__ __launch_bounds__(256) void qline_kernel(QLParams q) {
       T fm3[NR], fm2[NR], fm1[NR], fp1[NR], fp2[NR], b1[NR];
 #pragma unroll
       for (int k = 0; k < NR; ++k) {
-        fm3[k] = fl[wave][nbr(i1[k], -3)];
-        fm2[k] = fl[wave][nbr(i1[k], -2)];
-        fm1[k] = fl[wave][nbr(i1[k], -1)];
-        fp1[k] = fl[wave][nbr(i1[k], 1)];
-        fp2[k] = fl[wave][nbr(i1[k], 2)];
+        fm3[k] = fl[wave][nm3[k]];
+        fm2[k] = fl[wave][nm2[k]];
+        fm1[k] = fl[wave][nm1[k]];
+        fp1[k] = fl[wave][np1[k]];
+        fp2[k] = fl[wave][np2[k]];
         b1[k] = ql_beta<R, T>(fm1[k], y[k][3], fp1[k]);
         if (live[k]) bl[wave][i1[k]] = b1[k];
       }
@@ -185,9 +198,9 @@ __global__ __launch_bounds__(256) void qline_kernel(QLParams q) {
       for (int k = 0; k < NR; ++k) {
         T bm = T(0), b0 = T(0), bp = T(0);
         if constexpr (R == RC_WENO5) {
-          bm = bl[wave][nbr(i1[k], -2)];
-          b0 = bl[wave][nbr(i1[k], -1)];
-          bp = bl[wave][nbr(i1[k], 1)];
+          bm = bl[wave][nm2[k]];
+          b0 = bl[wave][nm1[k]];
+          bp = bl[wave][np1[k]];
         }
         const T s = ql_lower_face<R, T>(fm3[k], fm2[k], fm1[k], y[k][3], fp1[k], fp2[k], bm, b0,
                                         b1[k], bp, pos1);
@@ -197,7 +210,7 @@ __global__ __launch_bounds__(256) void qline_kernel(QLParams q) {
       __builtin_amdgcn_wave_barrier();
 #pragma unroll
       for (int k = 0; k < NR; ++k) {
-        const T su = sl[wave][nbr(i1[k], 1)];
+        const T su = sl[wave][np1[k]];
         d[k] = -(C1 * su - C1 * fm3[k]) * tinv1 - d[k];
       }
       __builtin_amdgcn_wave_barrier();
@@ -206,20 +219,20 @@ __global__ __launch_bounds__(256) void qline_kernel(QLParams q) {
       for (int k = 0; k < NR; ++k) d[k] = -d[k];
     }
     // source, RK combination, store
-    const long rb = rowbase(r);
+    T* orow = outv + rcur * rowP;
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
       const T f = y[k][3];
       T dk = d[k];
-      if (q.source) {
+      if (src) {
         T fmb = tc1[k];
-        if (q.p_dim >= 2) fmb = fmb * tc2[k];
-        if (q.p_dim >= 3) fmb = fmb * tc3[k];
+        if (src2) fmb = fmb * tc2[k];
+        if (src3) fmb = fmb * tc3[k];
         dk = dk + (fmb - f) * tinvtau;
       }
       const T tv = (w_is_u ? f : wc[k]) + dk * tcdt;
       const T x = use_un ? ta * unc[k] + tb * tv : (q.b == 1.0 ? tv : tb * tv);
-      if (live[k]) outv[rb + i1[k]] = x;
+      if (live[k]) orow[i1[k]] = x;
       // slide
 #pragma unroll
       for (int j = 0; j < 6; ++j) y[k][j] = y[k][j + 1];
@@ -233,12 +246,16 @@ __global__ __launch_bounds__(256) void qline_kernel(QLParams q) {
       tc2[k] = tn2[k];
       tc3[k] = tn3[k];
     }
+    rcur = rnx1;
+    rnx1 = rnx1 + 1 == n2 ? 0 : rnx1 + 1;
+    rnx4 = rnx4 + 1 == n2 ? 0 : rnx4 + 1;
   }
 }
 
 // Moments of f in the q1-line layout: workgroup (q2 row, 64-wide q1 block,
-// p chunk); wave w sums p = chunk start + w, + 4, ... for its 64 q1 cells;
-// the four waves are added in a fixed order -> part[chunk][5][Q].
+// p chunk); wave w sums a contiguous quarter of the chunk's p range for its
+// 64 q1 cells (loads issued eight at a time), and the four waves are added in
+// a fixed order -> part[chunk][5][Q].
 template <typename T>
 __global__ __launch_bounds__(256) void mline_kernel(const T* f, int n1, int n2, int P, int nchunk,
                                                     int n_p1, int n_p2, double p1s, double dp1,
@@ -254,21 +271,37 @@ __global__ __launch_bounds__(256) void mline_kernel(const T* f, int n1, int n2, 
   const int chunk = b / n2;
   const int i1 = blk * 64 + lane;
   const bool live = i1 < n1;
-  const int cp = (P + nchunk - 1) / nchunk;
-  const int pstart = chunk * cp, pend = min(P, pstart + cp);
-  const T* row = f + (long)i2 * P * n1 + (live ? i1 : 0);
+  const int per = (P + 4 * nchunk - 1) / (4 * nchunk);
+  const int pa = min(P, (chunk * 4 + wave) * per), pe = min(P, pa + per);
+  const T* col = f + (long)i2 * P * n1 + (live ? i1 : 0);
   double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
-  for (int p = pstart + wave; p < pend; p += 4) {
-    const int j1 = p % n_p1, j23 = p / n_p1;
-    const double v1 = centre(p1s, j1, dp1), v2 = centre(p2s, j23 % n_p2, dp2);
-    const double v3 = centre(p3s, j23 / n_p2, dp3);
-    const double e = 0.5 * ((v1 * v1 + v2 * v2) + v3 * v3);
-    const double x = live ? double(row[(long)p * n1]) : 0.0;
-    a0 += x;
-    a1 += x * v1;
-    a2 += x * v2;
-    a3 += x * v3;
-    a4 += x * e;
+  int j1 = pa % n_p1, j2 = (pa / n_p1) % n_p2, j3 = pa / (n_p1 * n_p2);
+  double v2 = centre(p2s, j2, dp2), v3 = centre(p3s, j3, dp3);
+  for (int p = pa; p < pe; p += 8) {
+    double x[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = (p + k < pe && live) ? double(col[(long)(p + k) * n1]) : 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (p + k < pe) {
+        const double v1 = centre(p1s, j1, dp1);
+        const double e = 0.5 * ((v1 * v1 + v2 * v2) + v3 * v3);
+        a0 += x[k];
+        a1 += x[k] * v1;
+        a2 += x[k] * v2;
+        a3 += x[k] * v3;
+        a4 += x[k] * e;
+        if (++j1 == n_p1) {
+          j1 = 0;
+          if (++j2 == n_p2) {
+            j2 = 0;
+            ++j3;
+            v3 = centre(p3s, j3, dp3);
+          }
+          v2 = centre(p2s, j2, dp2);
+        }
+      }
+    }
   }
   red[0][wave][lane] = a0;
   red[1][wave][lane] = a1;

@@ -50,7 +50,7 @@ def test_invalid_arguments_are_rejected():
     g2.n_p1, g2.layout = 48, 1
     tab = (48 + 48 + 1) * 256 * 8
     assert lib.boltx_workspace_size(ctypes.byref(g2), ctypes.byref(ph)) == \
-        (tab + 255) // 256 * 256 + 4 * 5 * 256 * 8
+        (tab + 255) // 256 * 256 + 8 * 5 * 256 * 8  # 8 p-chunks of moment partials
     g2.n_q1 = 512
     rc = lib.boltx_moments(ctypes.byref(g2), ctypes.byref(ph), ctypes.c_void_p(1),
                            ctypes.c_void_p(1), ctypes.c_void_p(1), None)
