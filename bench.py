@@ -27,21 +27,6 @@ METRIC = ("phase-space cell-updates/s, FP32-bulk vs FP64 at rel-L2≤1e-5; "
           "% HBM peak")
 
 
-def _stage_timer(nls, events):
-    """Wrap nls._stage so each stage launch is bracketed by HIP events."""
-    import torch
-    inner = nls._stage
-
-    def timed(*a, **k):
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record()
-        inner(*a, **k)
-        e1.record()
-        events.append((e0, e1))
-    nls._stage = timed
-
-
 def run_gpu(precision, steps, warmup):
     import torch
     from tests.cases import case_2d2v, hip_solver
@@ -51,7 +36,7 @@ def run_gpu(precision, steps, warmup):
         nls._advance(c.dt)
     nls.check_divergence()
     events = []
-    _stage_timer(nls, events)
+    nls._stage_events = events
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -59,19 +44,34 @@ def run_gpu(precision, steps, warmup):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     nls.check_divergence()
+    nls._stage_events = None
     stage_ms = [e0.elapsed_time(e1) for e0, e1 in events]
     cells = nls._Q * nls._P
     bytes_per_cell_step = 8 * (8 if precision == 'fp64' else 4)
     kernel_s = sum(stage_ms) / 1e3
     achieved = bytes_per_cell_step * cells * steps / kernel_s / 1e9
+    traffic = _traffic(precision)
     return nls, {
         "cells": cells,
         "ms_per_step": wall / steps * 1e3,
         "value": cells * steps / wall,
         "stage_launch_ms_avg": sum(stage_ms) / len(stage_ms),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "kernel": "qline_kernel (stage stencil; SSP-RK3 stages average 8/3 f-passes)",
     }
+
+
+def _traffic(precision):
+    """HBM bytes per stage-kernel launch measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration); the
+    measured figures are kept in profiles/traffic.json."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            return json.load(fh).get(precision, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
 
 
 def cpu_baseline():
@@ -93,13 +93,36 @@ def cpu_baseline():
                       "64^2 x 48^2 cells, 1 step (%.1f s)" % dt}
 
 
+def probe(precision):
+    """Launch sequence for counter runs: three copies of a known size, then one
+    SSP-RK3 step (three stage kernels and their moment passes)."""
+    import torch
+    from tests.cases import case_2d2v, hip_solver
+    x = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")  # 256 MiB
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    c = case_2d2v(seed=0, precision=precision)
+    nls = hip_solver(c)
+    nls._advance(c.dt)
+    torch.cuda.synchronize()
+    print("probe done", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--probe", choices=["fp64", "fp32-bulk"],
+                    help="traffic probe for rocprofv3 --pmc: a 256 MiB device copy "
+                         "(the counter calibration) and then one step of the workload")
     args = ap.parse_args()
+    if args.probe:
+        probe(args.probe)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the step shards over q with a halo exchange per stage)", file=sys.stderr)

@@ -115,6 +115,8 @@ class nonlinear_solver(object):
         self._source = ps.source
         self._initialize(p)
         self._timings = {'time_ts': 0.0, 'steps': 0}
+        # list of (start, end) HIP events around every stage-kernel call, or None
+        self._stage_events = None
 
     # -- configuration ------------------------------------------------------
     def _configure(self, ps, p):
@@ -295,11 +297,27 @@ class nonlinear_solver(object):
                                                   _lib.ptr(J), _lib.stream_handle()))
                 fs.evolve_electrodynamic_fields(J, self.dt)
             F = [fs._take_for_C_p() for _ in range(3)]  # df_dt_fvm.py:251-267
+        # In the q1-line layout the moments of the output are a separate
+        # streaming pass; issue it as its own call.
+        split = self._layout == _lib.LAYOUT['q1-lines'] and mom_out is not None
         st = _lib.Stage(_lib.ptr(u), _lib.ptr(w), _lib.ptr(un), _lib.ptr(out),
-                        float(a), float(b), float(cdt), _lib.ptr(mom_in), _lib.ptr(mom_out),
+                        float(a), float(b), float(cdt), _lib.ptr(mom_in),
+                        None if split else _lib.ptr(mom_out),
                         _lib.ptr(F[0]), _lib.ptr(F[1]), _lib.ptr(F[2]), _lib.ptr(self._ws))
+        ev = self._stage_events
+        if ev is not None:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e0.record()
         _lib.check(self._L.boltx_fvm_stage(ctypes.byref(self._grid), ctypes.byref(self._phys),
                                            ctypes.byref(st), _lib.stream_handle()))
+        if ev is not None:
+            e1 = torch.cuda.Event(enable_timing=True)
+            e1.record()
+            ev.append((e0, e1))
+        if split:
+            _lib.check(self._L.boltx_moments(ctypes.byref(self._grid), ctypes.byref(self._phys),
+                                             _lib.ptr(out), _lib.ptr(mom_out), _lib.ptr(self._ws),
+                                             _lib.stream_handle()))
 
     def _advance(self, dt):
         """Advance f by dt without the host-side divergence check (the device
