@@ -268,3 +268,156 @@ def test_c1_runs_and_conserves_mass():
     for _ in range(20):
         o.strang_timestep(c.dt)
     assert abs(float(o.interior().sum()) - n0) <= 1e-12 * n0
+
+
+def test_minmod_mc_slope_on_random_line():
+    """minmod.py:18-62: slope = minmod(2 (f_i - f_{i-1}), (f_{i+1} - f_{i-1}) / 2,
+    2 (f_{i+1} - f_i)) on a periodic line, checked against a numpy evaluation of
+    the monotonised-central limiter written from its definition."""
+    v = rng.normal(size=400)
+    back = v - np.roll(v, 1)
+    fwd = np.roll(v, -1) - v
+    cands = np.stack([2 * back, 0.5 * (back + fwd), 2 * fwd])
+    agree = (np.all(cands > 0, axis=0)) | (np.all(cands < 0, axis=0))
+    slope = np.where(agree, np.sign(cands[0]) * np.abs(cands).min(axis=0), 0.0)
+    left, right = bo.reconstruct_minmod(_line(v), 2)
+    assert np.allclose(right.reshape(-1).numpy(), v + 0.5 * slope, rtol=0, atol=1e-14)
+    assert np.allclose(left.reshape(-1).numpy(), v - 0.5 * slope, rtol=0, atol=1e-14)
+
+
+def test_weno5_ideal_weights_exact_on_quartics(monkeypatch):
+    """With every smoothness indicator equal, the weights are the ideal ones and
+    each face is the 5-point Lagrange interpolant of the stencil: exact for any
+    quartic.  Pins the ideal weights and every substencil coefficient."""
+    monkeypatch.setattr(bo, "weno5_beta", lambda a, b, c: torch.ones_like(a))
+    cf = rng.normal(size=(5, 300))
+    xs = np.arange(-2, 3)[:, None].astype(np.float64)
+    y = sum(cf[k] * xs**k for k in range(5))
+    L, R = bo.weno5_faces(*[T(v) for v in y])
+    P = lambda x: sum(cf[k] * x**k for k in range(5))
+    assert np.allclose(R.numpy(), P(0.5), rtol=0, atol=1e-12 * np.abs(cf).sum(axis=0).max())
+    assert np.allclose(L.numpy(), P(-0.5), rtol=0, atol=1e-12 * np.abs(cf).sum(axis=0).max())
+
+
+# -- FDTD ---------------------------------------------------------------------
+def _fields_oracle(N):
+    c = maxwellian_case(N_q1=N, N_q2=N, N_p1=1, N_ghost=3, p_dim=1, recon_q="minmod",
+                        recon_p="minmod", source=False, fields=True)
+    return oracle_solver(c).fields_solver
+
+
+def _interior_yee(fs, comp):
+    g1, g2 = fs.N_g1, fs.N_g2
+    return fs.yee_grid_EM_fields[comp, 0, g1:g1 + fs.N_q1, g2:g2 + fs.N_q2]
+
+
+def _div_b(fs):
+    """Discrete divergence at the Yee nodes (i, j): B1 lives at (i+1/2, j)
+    and B2 at (i, j+1/2)."""
+    B1, B2 = _interior_yee(fs, 3), _interior_yee(fs, 4)
+    return (B1 - torch.roll(B1, 1, 0)) / fs.dq1 + (B2 - torch.roll(B2, 1, 1)) / fs.dq2
+
+
+def test_fdtd_preserves_div_b():
+    """fdtd_explicit.py:84-138: the discrete curl of the Yee update has zero
+    discrete divergence, so div B is constant to rounding for any E, B, J
+    (test_check_maxwells_contraints.py:96 holds it to 1e-14)."""
+    fs = _fields_oracle(16)
+    gen = torch.Generator().manual_seed(3)
+    fs.yee_grid_EM_fields = fs._bc(torch.rand(fs.yee_grid_EM_fields.shape, dtype=F64,
+                                              generator=gen))
+    d0 = _div_b(fs)
+    sh = (1, 1) + tuple(fs.yee_grid_EM_fields.shape[2:])
+    for _ in range(10):
+        fs.J1, fs.J2, fs.J3 = (torch.rand(sh, dtype=F64, generator=gen) for _ in range(3))
+        fs.fdtd(0.01)
+    drift = float((_div_b(fs) - d0).abs().max())
+    assert drift <= 1e-12 * float(d0.abs().max()), drift
+
+
+def _plane_wave_error(N):
+    """E3 = cos(k.q - w t), B1 = (k2/w) cos(.), B2 = -(k1/w) cos(.) solves the
+    vacuum Maxwell equations (eps = mu = 1); E starts at t = 0 and B at dt/2
+    as the leapfrog expects.  Returns max |E3 - exact| at t = 0.25."""
+    fs = _fields_oracle(N)
+    k1 = k2 = 2 * math.pi
+    w = math.hypot(k1, k2)
+    dt = 0.2 / N
+    n = int(round(0.25 / dt))
+
+    def cos(q1, q2, t):
+        return torch.cos(k1 * q1 + k2 * q2 - w * t)[0]
+    y = torch.zeros_like(fs.yee_grid_EM_fields)
+    y[2] = cos(fs.q1_center, fs.q2_center, 0.0)
+    y[3] = k2 / w * cos(fs.q1_center_bot, fs.q2_center_bot, dt / 2)
+    # B2 sits at (i, j+1/2); the solver's q2_left_center is q2_left_bot
+    # (calculate_q.py:37), so take q2 from the cell centres instead
+    y[4] = -k1 / w * cos(fs.q1_left_center, fs.q2_center, dt / 2)
+    fs.yee_grid_EM_fields = y
+    fs.J1 = fs.J2 = fs.J3 = torch.zeros((1, 1) + tuple(y.shape[2:]), dtype=F64)
+    for _ in range(n):
+        fs.fdtd(dt)
+    g1, g2 = fs.N_g1, fs.N_g2
+    exact = cos(fs.q1_center, fs.q2_center, n * dt)[0, g1:g1 + N, g2:g2 + N]
+    return float((_interior_yee(fs, 2) - exact).abs().max())
+
+
+def test_fdtd_plane_wave_second_order():
+    """Mode test in the style of test_fdtd_explicit.py:120-513: the error of a
+    travelling plane wave falls with slope -2 under dt proportional to dq."""
+    Ns = [16, 32, 64]
+    errs = [_plane_wave_error(N) for N in Ns]
+    slope = np.polyfit(np.log(Ns), np.log(errs), 1)[0]
+    assert abs(slope + 2) < 0.25, (errs, slope)
+
+
+# -- the pins must catch planted mistakes ---------------------------------------
+def _mutant(fn, old, new):
+    """A copy of oracle function ``fn`` with the source text ``old`` replaced
+    by ``new`` (the planted mistake), compiled in the oracle's namespace."""
+    import inspect
+    import textwrap
+    src = textwrap.dedent(inspect.getsource(fn))
+    assert old in src, old
+    ns = dict(vars(bo))
+    exec(compile(src.replace(old, new, 1), "<mutant>", "exec"), ns)
+    return ns[fn.__name__]
+
+
+@pytest.mark.parametrize("target,old,new,pins", [
+    ("af_shift", "shifts=ks", "shifts=[-k for k in ks]",
+     ["test_af_shift_direction_and_wrap", "test_upwind_piecewise_constant_is_first_order_upwind"]),
+    ("weno5_beta", "(10 / 3) * c * c", "(10 / 3) * b * c",
+     ["test_weno5_beta_expanded_equals_jiang_shu"]),
+    ("weno5_faces", "w2r = 5 / (8 *", "w2r = 5 / (16 *",
+     ["test_weno5_ideal_weights_exact_on_quartics"]),
+    ("weno5_faces", "u3l = 1.875 * y2", "u3l = 1.75 * y2",
+     ["test_weno5_exact_on_quadratic_point_data"]),
+    ("slope_minmod", "theta = 2\n", "theta = 1\n", ["test_minmod_mc_slope_on_random_line"]),
+    ("ppm_faces", "(DQ3 - DQ2) / 6", "(DQ3 - DQ1) / 6", ["test_ppm_linear_constant_and_mirror"]),
+    ("BGK_moments", "(1 / params.p_dim)", "(1 / (params.p_dim + 1))",
+     ["test_bgk_conserves_moments_and_vanishes_on_maxwellian"]),
+    ("f0", "(m / (2 * math.pi * k * T))\n", "(m / (math.pi * k * T))\n",
+     ["test_bgk_conserves_moments_and_vanishes_on_maxwellian"]),
+])
+def test_pins_catch_planted_mistakes(monkeypatch, target, old, new, pins):
+    monkeypatch.setattr(bo, target, _mutant(getattr(bo, target), old, new))
+    g = globals()
+    for name in pins:
+        kwargs = {"monkeypatch": monkeypatch} if "monkeypatch" in g[name].__code__.co_varnames else {}
+        with pytest.raises(AssertionError):
+            g[name](**kwargs)
+
+
+def test_fdtd_pins_catch_planted_mistakes(monkeypatch):
+    bad_b = _mutant(bo.OracleFields.fdtd_evolve_B, "E3_minus_q2 = af_shift(E3, 0, 0, 0, 1)",
+                    "E3_minus_q2 = af_shift(E3, 0, 0, 0, -1)")
+    monkeypatch.setattr(bo.OracleFields, "fdtd_evolve_B", bad_b)
+    with pytest.raises(AssertionError):
+        test_fdtd_preserves_div_b()
+    monkeypatch.undo()
+    bad_e = _mutant(bo.OracleFields.fdtd_evolve_E, "- (B1_plus_q2 - B1) / self.dq2",
+                    "+ (B1_plus_q2 - B1) / self.dq2")
+    monkeypatch.setattr(bo.OracleFields, "fdtd_evolve_E", bad_e)
+    with pytest.raises(AssertionError):
+        test_fdtd_plane_wave_second_order()
