@@ -16,7 +16,8 @@ LAYOUT = {"reference": 0, "q1-lines": 1}
 
 EXPORTS = ("boltx_fvm_stage", "boltx_moments", "boltx_workspace_size", "boltx_workspace_init",
            "boltx_currents", "boltx_fdtd_step", "boltx_yee_to_cell_centered",
-           "boltx_check_finite", "boltx_version", "boltx_last_error")
+           "boltx_check_finite", "boltx_adjoint_stage", "boltx_adjoint_workspace_size",
+           "boltx_version", "boltx_last_error")
 MOM_ROWS = 12
 
 
@@ -48,6 +49,13 @@ class Stage(ctypes.Structure):
                 ("fields_c_p3", ctypes.c_void_p), ("workspace", ctypes.c_void_p)]
 
 
+class Adjoint(ctypes.Structure):
+    _fields_ = [("u", ctypes.c_void_p), ("mom_u", ctypes.c_void_p), ("lam", ctypes.c_void_p),
+                ("base", ctypes.c_void_p), ("out", ctypes.c_void_p),
+                ("a", ctypes.c_double), ("b", ctypes.c_double), ("c", ctypes.c_double),
+                ("tau_grad", ctypes.c_void_p), ("workspace", ctypes.c_void_p)]
+
+
 class BoltxError(RuntimeError):
     pass
 
@@ -76,9 +84,12 @@ def load(path=LIB_PATH):
     lib.boltx_fdtd_step.argtypes = [g, vp, vp, dbl, dbl, dbl, vp]
     lib.boltx_yee_to_cell_centered.argtypes = [g, vp, vp, vp]
     lib.boltx_check_finite.argtypes = [g, vp, vp, vp]
+    lib.boltx_adjoint_stage.argtypes = [g, ph, ctypes.POINTER(Adjoint), vp]
+    lib.boltx_adjoint_workspace_size.argtypes = [g, ph]
     for name in EXPORTS[:-2]:
         getattr(lib, name).restype = ctypes.c_int
     lib.boltx_workspace_size.restype = ctypes.c_size_t
+    lib.boltx_adjoint_workspace_size.restype = ctypes.c_size_t
     lib.boltx_version.argtypes = []
     lib.boltx_version.restype = ctypes.c_int
     lib.boltx_last_error.argtypes = []

@@ -10,12 +10,14 @@ import sys
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
-SRC = os.path.join(PKG, "csrc", "boltx.hip")
+CSRC = os.path.join(PKG, "csrc")
+# translation units, compiled in parallel and linked into one library
+SOURCES = [os.path.join(CSRC, n) for n in ("boltx.hip", "ql_f64.hip", "ql_f32.hip")]
 OUT = os.path.join(PKG, "libboltx_hip.so")
 
 # -packed-fp32-ops: no v_pk_*_f32 in generated code (scalar FP32 only).
 HIPCC_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-ffp-contract=fast-honor-pragmas",
     "-I", os.path.join(ROOT, "include"),
@@ -29,19 +31,37 @@ def hipcc():
     raise RuntimeError("hipcc not found: the HIP library cannot be built")
 
 
-def build(verbose=False):
-    """Compile ``libboltx_hip.so`` (always recompiles)."""
-    tmp = OUT + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp, SRC]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
+def _filter(stderr):
     noise = "is not a recognized feature for this target"
-    err = "\n".join(l for l in res.stderr.splitlines() if noise not in l and l.strip())
+    return "\n".join(l for l in stderr.splitlines() if noise not in l and l.strip())
+
+
+def build(verbose=False):
+    """Compile ``libboltx_hip.so`` (always recompiles every unit)."""
+    objs, procs = [], []
+    for src in SOURCES:
+        obj = OUT + "." + os.path.splitext(os.path.basename(src))[0] + ".o"
+        cmd = [hipcc()] + HIPCC_FLAGS + ["-c", "-o", obj, src]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((src, subprocess.Popen(cmd, stderr=subprocess.PIPE, text=True)))
+        objs.append(obj)
+    failed = []
+    for src, p in procs:
+        err = _filter(p.communicate()[1])
+        if p.returncode != 0:
+            failed.append(f"{src}:\n{err}")
+        elif err and verbose:
+            print(err, file=sys.stderr)
+    if failed:
+        raise RuntimeError("hipcc failed:\n" + "\n".join(failed))
+    tmp = OUT + ".tmp"
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + err)
-    if err and verbose:
-        print(err, file=sys.stderr)
+        raise RuntimeError("link failed:\n" + _filter(res.stderr))
+    for o in objs:
+        os.remove(o)
     os.replace(tmp, OUT)
     return OUT
 

@@ -1,0 +1,280 @@
+// Adjoint (vector-Jacobian product) of one FVM stage, FP64, fields off.
+//
+// The stage is out = a*un + b*(w + cdt*L(u)) with
+//   L(u) = -(F1(i+1/2) - F1(i-1/2))/dq1 - (F2(j+1/2) - F2(j-1/2))/dq2 + (f_MB(u) - u)/tau
+// (df_dt_fvm.py:56-97, collision_operator.py:70-128), where each face flux is
+// C times the upwind face state (riemann.py:46-49; LLF has the same
+// derivative for a linear flux).  Given mu (the adjoint of the stage output
+// seen through L), adj_stage_kernel writes
+//   out = a*base + b*mu + c*J(u)^T mu,
+// with J = dL/du, and adj_bgk_reduce_kernel the per-q-cell pieces of the
+// collision term's Jacobian and c * sum_p mu dL/dtau.
+//
+// Advection: sum_i mu_i L_i = sum_k F_k (mu_k - mu_{k-1}) / dq over the lower
+// faces k, so the face state s_k has adjoint C (mu_k - mu_{k-1}) / dq, and the
+// gather for cell j visits the five faces whose stencils contain j.  Each
+// face's derivative with respect to cell j is the reference reconstruction
+// (fvm_math.h, the same templates the forward kernels use) evaluated on
+// one-direction dual numbers.
+//
+// BGK: f_MB = n (m/2 pi k T)^(d/2) prod_{k<=d} exp(-m (v_k-u_k)^2 / 2kT) with
+// n = M0, u_k = M_k/(n+eps), T = (2 E m - n m |u|^2)/(d (n+eps)) + eps
+// (collision_operator.py:102-116) and the raw moments M = sum_p f w_p dV.  The
+// reduction over p of mu f_MB, mu f_MB (v-u), mu f_MB |v-u|^2 gives the
+// derivatives with respect to n, u, T, which the chain rule through
+// u(M), T(M) turns into one weight per moment; the pointwise pass then adds
+// dV (g0 + g1 v1 + g2 v2 + g3 v3 + g4 |v|^2/2) / tau - mu / tau.
+#pragma once
+#include "../../include/boltx.h"
+#include "fvm_math.h"
+
+namespace bx {
+
+// Forward-mode dual number: value and one directional derivative.
+struct Dual {
+  double v, d;
+  __host__ __device__ Dual() : v(0.0), d(0.0) {}
+  __host__ __device__ Dual(double x) : v(x), d(0.0) {}
+  __host__ __device__ Dual(double x, double dx) : v(x), d(dx) {}
+};
+__host__ __device__ inline Dual operator+(Dual a, Dual b) { return Dual(a.v + b.v, a.d + b.d); }
+__host__ __device__ inline Dual operator-(Dual a, Dual b) { return Dual(a.v - b.v, a.d - b.d); }
+__host__ __device__ inline Dual operator-(Dual a) { return Dual(-a.v, -a.d); }
+__host__ __device__ inline Dual operator*(Dual a, Dual b) {
+  return Dual(a.v * b.v, a.d * b.v + a.v * b.d);
+}
+__host__ __device__ inline Dual operator/(Dual a, Dual b) {
+  const double r = 1.0 / b.v;
+  return Dual(a.v * r, (a.d - a.v * r * b.d) * r);
+}
+__host__ __device__ inline bool operator<(Dual a, Dual b) { return a.v < b.v; }
+__host__ __device__ inline bool operator>(Dual a, Dual b) { return a.v > b.v; }
+__host__ __device__ inline bool operator<=(Dual a, Dual b) { return a.v <= b.v; }
+__host__ __device__ inline bool operator>=(Dual a, Dual b) { return a.v >= b.v; }
+__host__ __device__ inline Dual aabs(Dual a) { return a.v < 0.0 ? -a : a; }
+__host__ __device__ inline Dual fast_rcp(Dual a) {
+  const double r = 1.0 / a.v;
+  return Dual(r, -a.d * r * r);
+}
+// weno5.py:58-71 (expanded indicator plus the eps floor) on duals
+template <>
+__host__ __device__ inline Dual weno_beta<Dual>(Dual a, Dual b, Dual c) {
+  const Dual s = Dual(4.0 / 3.0) * a * a - Dual(19.0 / 3.0) * a * b + Dual(25.0 / 3.0) * b * b +
+                 Dual(11.0 / 3.0) * a * c - Dual(31.0 / 3.0) * b * c + Dual(10.0 / 3.0) * c * c;
+  return s + Dual(1e-17) * (Dual(1.0) + aabs(a) + aabs(b) + aabs(c));
+}
+
+// Derivative of the face state of the centre cell w[s+2] of the window
+// w[s..s+4] (its right face if ``right``) with respect to the window entry
+// w[4].
+template <int R>
+__host__ __device__ inline double face_partial(const double* w, int s, bool right) {
+  Dual y[5];
+#pragma unroll
+  for (int r = 0; r < 5; ++r) y[r] = Dual(w[s + r], s + r == 4 ? 1.0 : 0.0);
+  return face_value<R, Dual>(y[0], y[1], y[2], y[3], y[4], right).d;
+}
+
+// Gather of the advection adjoint along one axis for the cell at window
+// centre: w[0..8] = u at offsets -4..4, m[0..6] = mu at offsets -3..3,
+// cdq = C/dq.  Upwind (C > 0): the lower face of cell k is the right face of
+// cell k-1, stencil k-3..k+1, for k = j-1..j+3; otherwise the left face of
+// cell k, stencil k-2..k+2, for k = j-2..j+2.
+template <int R>
+__host__ __device__ inline double axis_adjoint(const double* w, const double* m, double cdq) {
+  double acc = 0.0;
+  if (cdq > 0.0) {
+#pragma unroll
+    for (int o = -1; o <= 3; ++o) {
+      const double sk = cdq * (m[o + 3] - m[o + 2]);
+      acc += sk * face_partial<R>(w, o + 1, true);
+    }
+  } else if (cdq < 0.0) {
+#pragma unroll
+    for (int o = -2; o <= 2; ++o) {
+      const double sk = cdq * (m[o + 3] - m[o + 2]);
+      acc += sk * face_partial<R>(w, o + 2, false);
+    }
+  }
+  return acc;
+}
+
+struct AdjParams {
+  int n1, n2, P, n_p1, n_p2, n_p3, layout, p_dim, source;
+  double inv_dq1, inv_dq2, inv_tau, measure, mass, kB;
+  double p1s, dp1, p2s, dp2, p3s, dp3;
+  const double* u;
+  const double* mom;  // moment block of u
+  const double* lam;
+  const double* base;
+  double* out;
+  double a, b, c;
+  double* gq;        // [5][Q] moment weights of the collision Jacobian
+  double* tau_grad;  // [Q] or null
+};
+
+__host__ __device__ inline long adj_index(const AdjParams& ap, int i1, int i2, int p) {
+  return ap.layout ? ((long)i2 * ap.P + p) * ap.n1 + i1 : ((long)i2 * ap.n1 + i1) * ap.P + p;
+}
+
+// Per q-cell: reductions over p and the chain rule to the raw moments.
+__host__ __device__ inline void adj_reduce_cell(const AdjParams& ap, int q) {
+  const int Q = ap.n1 * ap.n2;
+  const int i1 = q % ap.n1, i2 = q / ap.n1;
+  const double* b = ap.mom + MOM_BGK * Q + q;
+  const double A = b[0], u1 = b[Q], u2 = b[2 * Q], u3 = b[3 * Q], T = b[4 * Q], n = b[5 * Q];
+  const double beta = b[6 * Q];
+  const int d = ap.p_dim;
+  double R0 = 0, R11 = 0, R12 = 0, R13 = 0, R2 = 0, S = 0;
+  int j1 = 0, j2 = 0, j3 = 0;
+  for (int p = 0; p < ap.P; ++p) {
+    const double v1 = centre(ap.p1s, j1, ap.dp1), v2 = centre(ap.p2s, j2, ap.dp2);
+    const double v3 = centre(ap.p3s, j3, ap.dp3);
+    const double d1 = v1 - u1, d2 = v2 - u2, d3 = v3 - u3;
+    double q2 = d1 * d1;
+    if (d >= 2) q2 += d2 * d2;
+    if (d >= 3) q2 += d3 * d3;
+    const double fmb = A * exp(-beta * q2);
+    const long idx = adj_index(ap, i1, i2, p);
+    const double mu = ap.lam[idx];
+    const double w = mu * fmb;
+    R0 += w;
+    R11 += w * d1;
+    R12 += w * d2;
+    R13 += w * d3;
+    R2 += w * q2;
+    S += mu * (fmb - ap.u[idx]);
+    if (++j1 == ap.n_p1) {
+      j1 = 0;
+      if (++j2 == ap.n_p2) {
+        j2 = 0;
+        ++j3;
+      }
+    }
+  }
+  const double eps = 1e-30, m = ap.mass, kB = ap.kB, ne = n + eps;
+  // derivatives of G = sum_p mu f_MB with respect to n, u_k, T
+  const double Gn = R0 / n;
+  const double mkT = m / (kB * T);
+  const double Gu1 = mkT * R11;
+  const double Gu2 = d >= 2 ? mkT * R12 : 0.0;
+  const double Gu3 = d >= 3 ? mkT * R13 : 0.0;
+  const double GT = -0.5 * d / T * R0 + 0.5 * mkT / T * R2;
+  // T(E, n, u): dT/dE, dT/du_k, dT/dn at fixed E and u
+  const double TE = 2.0 * m / (d * ne);
+  const double usq = u1 * u1 + u2 * u2 + u3 * u3;
+  const double Tn = -m * usq / (d * ne) - (T - eps) / ne;
+  const double Tu = -2.0 * n * m / (d * ne);
+  const double H1 = Gu1 + GT * Tu * u1, H2 = Gu2 + GT * Tu * u2, H3 = Gu3 + GT * Tu * u3;
+  double* g = ap.gq + q;
+  g[0] = Gn + GT * Tn - (H1 * u1 + H2 * u2 + H3 * u3) / ne;
+  g[Q] = H1 / ne;
+  g[2 * Q] = H2 / ne;
+  g[3 * Q] = H3 / ne;
+  g[4 * Q] = GT * TE;
+  // dL/dtau = -(f_MB - f)/tau^2
+  if (ap.tau_grad) ap.tau_grad[q] = -ap.c * ap.inv_tau * ap.inv_tau * S;
+}
+
+#ifndef BX_NO_KERNELS  // host-only builds of the per-cell code leave the kernels out
+__global__ void adj_bgk_reduce_kernel(AdjParams ap) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < ap.n1 * ap.n2) adj_reduce_cell(ap, q);
+}
+#endif
+
+// Cell t (memory order): out = a*base + b*mu + c*(J^T mu).
+template <int R>
+__host__ __device__ inline void adj_stage_cell(const AdjParams& ap, long t) {
+  const long Qn = (long)ap.n1 * ap.n2;
+  int i1, i2, p;
+  if (ap.layout) {
+    i1 = (int)(t % ap.n1);
+    const long r = t / ap.n1;
+    p = (int)(r % ap.P);
+    i2 = (int)(r / ap.P);
+  } else {
+    p = (int)(t % ap.P);
+    const long r = t / ap.P;
+    i1 = (int)(r % ap.n1);
+    i2 = (int)(r / ap.n1);
+  }
+  const int j1 = p % ap.n_p1, j2 = (p / ap.n_p1) % ap.n_p2, j3 = p / (ap.n_p1 * ap.n_p2);
+  const double v1 = centre(ap.p1s, j1, ap.dp1), v2 = centre(ap.p2s, j2, ap.dp2);
+  const double v3 = centre(ap.p3s, j3, ap.dp3);
+  const double mu = ap.lam[t];
+  double acc = 0.0;
+  double w[9], m[7];
+  if (ap.n1 > 1) {
+#pragma unroll
+    for (int r = 0; r < 9; ++r) w[r] = ap.u[adj_index(ap, qm_mod(i1 + r - 4, ap.n1), i2, p)];
+#pragma unroll
+    for (int r = 0; r < 7; ++r) m[r] = ap.lam[adj_index(ap, qm_mod(i1 + r - 3, ap.n1), i2, p)];
+    acc += axis_adjoint<R>(w, m, v1 * ap.inv_dq1);
+  }
+  if (ap.n2 > 1) {
+#pragma unroll
+    for (int r = 0; r < 9; ++r) w[r] = ap.u[adj_index(ap, i1, qm_mod(i2 + r - 4, ap.n2), p)];
+#pragma unroll
+    for (int r = 0; r < 7; ++r) m[r] = ap.lam[adj_index(ap, i1, qm_mod(i2 + r - 3, ap.n2), p)];
+    acc += axis_adjoint<R>(w, m, v2 * ap.inv_dq2);
+  }
+  if (ap.source) {
+    const int q = i2 * ap.n1 + i1;
+    const double* g = ap.gq + q;
+    const double e = 0.5 * ((v1 * v1 + v2 * v2) + v3 * v3);
+    acc += ap.inv_tau * (ap.measure * (g[0] + g[Qn] * v1 + g[2 * Qn] * v2 + g[3 * Qn] * v3 +
+                                       g[4 * Qn] * e) - mu);
+  }
+  double x = ap.b * mu + ap.c * acc;
+  if (ap.base) x += ap.a * ap.base[t];
+  ap.out[t] = x;
+}
+
+#ifndef BX_NO_KERNELS
+template <int R>
+__global__ __launch_bounds__(256) void adj_stage_kernel(AdjParams ap) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < (long)ap.n1 * ap.n2 * ap.P) adj_stage_cell<R>(ap, t);
+}
+#endif
+
+// Parameters of one adjoint stage from the C-ABI structs.
+__host__ inline AdjParams adj_params(const boltx_grid* g, const boltx_physics* ph,
+                                     const boltx_adjoint* adj) {
+  AdjParams ap;
+  ap.n1 = g->n_q1;
+  ap.n2 = g->n_q2;
+  ap.n_p1 = g->n_p1;
+  ap.n_p2 = g->n_p2;
+  ap.n_p3 = g->n_p3;
+  ap.P = g->n_p1 * g->n_p2 * g->n_p3;
+  ap.layout = g->layout;
+  ap.p_dim = ph->p_dim;
+  ap.source = ph->source_enabled;
+  ap.inv_dq1 = 1.0 / g->dq1;
+  ap.inv_dq2 = 1.0 / g->dq2;
+  ap.inv_tau = ph->source_enabled ? 1.0 / ph->tau : 0.0;
+  ap.measure = g->dp1 * g->dp2 * g->dp3;
+  ap.mass = ph->mass;
+  ap.kB = ph->boltzmann_constant;
+  ap.p1s = g->p1_start;
+  ap.dp1 = g->dp1;
+  ap.p2s = g->p2_start;
+  ap.dp2 = g->dp2;
+  ap.p3s = g->p3_start;
+  ap.dp3 = g->dp3;
+  ap.u = (const double*)adj->u;
+  ap.mom = adj->mom_u;
+  ap.lam = (const double*)adj->lam;
+  ap.base = (const double*)adj->base;
+  ap.out = (double*)adj->out;
+  ap.a = adj->a;
+  ap.b = adj->b;
+  ap.c = adj->c;
+  ap.gq = (double*)adj->workspace;
+  ap.tau_grad = adj->tau_grad;
+  return ap;
+}
+
+}  // namespace bx

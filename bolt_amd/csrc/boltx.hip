@@ -16,7 +16,9 @@
 #include "../../include/boltx.h"
 #include "fvm_stage.h"
 #include "qcell.h"
-#include "qline.h"
+#include "ql_launch.h"
+#include "qline_moments.h"
+#include "adjoint.h"
 
 namespace bx {
 
@@ -441,22 +443,21 @@ static size_t ql_workspace_bytes(const boltx_grid* g, int precision) {
 template <typename T>
 static int ql_moments(const boltx_grid* g, const StageParams& sp, const void* f, double* mom,
                       double* part, hipStream_t s) {
-  const int nblk = (g->n_q1 + 63) / 64;
-  hipLaunchKernelGGL(mline_kernel<T>, dim3(nblk * g->n_q2 * ML_CHUNKS), dim3(256), 0, s,
-                     (const T*)f, g->n_q1, g->n_q2, sp.P, ML_CHUNKS, g->n_p1, g->n_p2,
-                     g->p1_start, g->dp1, g->p2_start, g->dp2, g->p3_start, g->dp3, part);
+  if (mline_vec_applies<T>(g->n_q1)) {
+    hipLaunchKernelGGL(mline_vec_kernel<T>, dim3(g->n_q2 * ML_CHUNKS), dim3(256), 0, s,
+                       (const T*)f, g->n_q1, g->n_q2, sp.P, ML_CHUNKS, g->n_p1, g->n_p2,
+                       g->p1_start, g->dp1, g->p2_start, g->dp2, g->p3_start, g->dp3, part);
+  } else {
+    const int nblk = (g->n_q1 + 63) / 64;
+    hipLaunchKernelGGL(mline_kernel<T>, dim3(nblk * g->n_q2 * ML_CHUNKS), dim3(256), 0, s,
+                       (const T*)f, g->n_q1, g->n_q2, sp.P, ML_CHUNKS, g->n_p1, g->n_p2,
+                       g->p1_start, g->dp1, g->p2_start, g->dp2, g->p3_start, g->dp3, part);
+  }
   int rc = launch_status("mline_kernel");
   if (rc) return rc;
   hipLaunchKernelGGL(ml_finalize_kernel, dim3((sp.Q + 255) / 256), dim3(256), 0, s, part,
                      ML_CHUNKS, sp.Q, sp.measure, mom, sp.mass, sp.kB, sp.p_dim);
   return launch_status("ml_finalize_kernel");
-}
-
-template <typename T, int R>
-static void launch_ql_nr(const QLParams& q, int nb, hipStream_t s) {
-  if (q.n1 <= 64) hipLaunchKernelGGL((qline_kernel<T, R, 1>), dim3(nb), dim3(256), 0, s, q);
-  else if (q.n1 <= 128) hipLaunchKernelGGL((qline_kernel<T, R, 2>), dim3(nb), dim3(256), 0, s, q);
-  else hipLaunchKernelGGL((qline_kernel<T, R, 4>), dim3(nb), dim3(256), 0, s, q);
 }
 
 template <typename T>
@@ -503,12 +504,12 @@ static int ql_stage(const boltx_grid* g, const boltx_physics* ph, const boltx_st
   q.dp3 = g->dp3;
   q.tab = tab;
   const int nb = (sp.P + 3) / 4 * q.nseg;
-  switch (ph->recon_q) {
-    case RC_PC: launch_ql_nr<T, RC_PC>(q, nb, s); break;
-    case RC_MINMOD: launch_ql_nr<T, RC_MINMOD>(q, nb, s); break;
-    case RC_PPM: launch_ql_nr<T, RC_PPM>(q, nb, s); break;
-    default: launch_ql_nr<T, RC_WENO5>(q, nb, s);
-  }
+  const bool w_is_u = st->w == st->u;
+  int mode = QL_GENERAL;
+  if (w_is_u && st->a != 0.0) mode = QL_WU_COMB;
+  else if (st->a == 0.0 && st->b == 1.0) mode = w_is_u ? QL_WU_PLAIN : QL_W_PLAIN;
+  const int pd = sp.source ? sp.p_dim : 0;
+  ql_launch<T>(q, ph->recon_q, nb, mode, pd, s);
   if ((rc = launch_status("qline_kernel"))) return rc;
   if (st->mom_out) return ql_moments<T>(g, sp, st->out, st->mom_out, part, s);
   return BOLTX_OK;
@@ -707,4 +708,45 @@ extern "C" int boltx_check_finite(const boltx_grid* grid, const double* mom, int
   hipLaunchKernelGGL(check_finite_kernel, dim3((Q + 255) / 256), dim3(256), 0, (hipStream_t)stream,
                      mom, Q, flag);
   return launch_status("check_finite_kernel");
+}
+
+// ---- adjoint stage -----------------------------------------------------------
+extern "C" size_t boltx_adjoint_workspace_size(const boltx_grid* grid, const boltx_physics*) {
+  if (!grid) return 0;
+  return (size_t)5 * grid->n_q1 * grid->n_q2 * sizeof(double);
+}
+
+extern "C" int boltx_adjoint_stage(const boltx_grid* grid, const boltx_physics* phys,
+                                   const boltx_adjoint* adj, void* stream) {
+  int rc = check_grid(grid);
+  if (rc) return rc;
+  if ((rc = check_physics(phys))) return rc;
+  if (!adj || !adj->u || !adj->lam || !adj->out) return fail(BOLTX_EINVAL, "adjoint pointers missing");
+  if (adj->out == adj->u || adj->out == adj->lam)
+    return fail(BOLTX_EINVAL, "out must not alias u or lam");
+  if (phys->precision != BOLTX_FP64) return fail(BOLTX_EUNSUPPORTED, "the adjoint is FP64 only");
+  if (phys->p_advection) return fail(BOLTX_EUNSUPPORTED, "the adjoint needs fields off");
+  if (phys->source_enabled && (!adj->mom_u || !adj->workspace))
+    return fail(BOLTX_EINVAL, "the source needs mom_u and the workspace");
+  const int need = recon_reach(phys->recon_q) + 1;
+  if ((grid->n_q1 > 1 && grid->n_q1 < need) || (grid->n_q2 > 1 && grid->n_q2 < need))
+    return fail(BOLTX_EINVAL, "q-grid smaller than the stencil reach");
+  const AdjParams ap = adj_params(grid, phys, adj);
+  hipStream_t s = (hipStream_t)stream;
+  const long Q = (long)ap.n1 * ap.n2;
+  if (ap.source) {
+    hipLaunchKernelGGL(adj_bgk_reduce_kernel, dim3((unsigned)((Q + 255) / 256)), dim3(256), 0, s,
+                       ap);
+    if ((rc = launch_status("adj_bgk_reduce_kernel"))) return rc;
+  }
+  const unsigned nb = (unsigned)((Q * ap.P + 255) / 256);
+  switch (phys->recon_q) {
+    case RC_PC: hipLaunchKernelGGL(adj_stage_kernel<RC_PC>, dim3(nb), dim3(256), 0, s, ap); break;
+    case RC_MINMOD:
+      hipLaunchKernelGGL(adj_stage_kernel<RC_MINMOD>, dim3(nb), dim3(256), 0, s, ap);
+      break;
+    case RC_PPM: hipLaunchKernelGGL(adj_stage_kernel<RC_PPM>, dim3(nb), dim3(256), 0, s, ap); break;
+    default: hipLaunchKernelGGL(adj_stage_kernel<RC_WENO5>, dim3(nb), dim3(256), 0, s, ap);
+  }
+  return launch_status("adj_stage_kernel");
 }

@@ -1,0 +1,7 @@
+// FP32-bulk specialisations of the q1-line stage kernel.
+#define BX_QL_INSTANTIATE
+#include "ql_launch.h"
+
+namespace bx {
+template void ql_launch<float>(const QLParams&, int, int, int, int, hipStream_t);
+}

@@ -142,6 +142,29 @@ int boltx_yee_to_cell_centered(const boltx_grid *grid, const double *yee,
 int boltx_check_finite(const boltx_grid *grid, const double *mom, int32_t *flag,
                        void *stream);
 
+/* Adjoint of one stage (FP64, fields off).  For the stage
+ * out = a*un + b*(w + cdt*L(u)) and an adjoint lam of its output, the caller
+ * passes c = b*cdt and gets out = a*base + b*lam + c*J(u)^T lam, J = dL/du,
+ * and, if tau_grad is non-NULL, tau_grad[q] = c * sum_p lam * dL/dtau for
+ * each q-cell (the BGK source's dependence on its collision time).  base may
+ * be NULL and may alias out; out must not alias u or lam.  The reference has
+ * no adjoint: this differentiates df_dt_fvm.py:33-97 with the BGK source
+ * collision_operator.py:70-128 and the moments moments.py:5-22. */
+typedef struct boltx_adjoint {
+  const void *u;          /* forward stage input */
+  const double *mom_u;    /* moment block of u (needed when the source is on) */
+  const void *lam;        /* adjoint of the stage output */
+  const void *base;       /* or NULL */
+  void *out;
+  double a, b, c;
+  double *tau_grad;       /* [Q] or NULL */
+  void *workspace;        /* boltx_adjoint_workspace_size() bytes */
+} boltx_adjoint;
+
+int boltx_adjoint_stage(const boltx_grid *grid, const boltx_physics *phys,
+                        const boltx_adjoint *adj, void *stream);
+size_t boltx_adjoint_workspace_size(const boltx_grid *grid, const boltx_physics *phys);
+
 int boltx_version(void);
 const char *boltx_last_error(void);
 

@@ -135,3 +135,24 @@ extern "C" int host_flux_pair(int recon, int precision, int riemann, const doubl
   }
   return 0;
 }
+
+#include "../../bolt_amd/csrc/adjoint.h"
+
+// The adjoint stage's per-cell code on host memory (both passes).
+extern "C" int host_adjoint_stage(const boltx_grid* g, const boltx_physics* ph,
+                                  const boltx_adjoint* adj) {
+  const AdjParams ap = adj_params(g, ph, adj);
+  const int Q = ap.n1 * ap.n2;
+  if (ap.source)
+    for (int q = 0; q < Q; ++q) adj_reduce_cell(ap, q);
+  const long n = (long)Q * ap.P;
+  for (long t = 0; t < n; ++t) {
+    switch (ph->recon_q) {
+      case 0: adj_stage_cell<RC_PC>(ap, t); break;
+      case 1: adj_stage_cell<RC_MINMOD>(ap, t); break;
+      case 2: adj_stage_cell<RC_PPM>(ap, t); break;
+      default: adj_stage_cell<RC_WENO5>(ap, t);
+    }
+  }
+  return 0;
+}

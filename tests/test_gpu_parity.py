@@ -207,3 +207,46 @@ def test_layouts_agree_full_size(gpu, precision, rq):
               f"moments {dm:.3e}", flush=True)
         assert d <= tol * ref
         assert dm <= (1e-13 if precision == "fp64" else 1e-6)
+
+
+QLINE_CASES = {
+    # name: (maxwellian_case kwargs, steps)
+    "2d2v_64x12": dict(N_q1=64, N_q2=12, N_p1=8, N_p2=8, p_dim=2, modes=((1, 1), (2, 1)),
+                       amps=(4e-4, 7e-4), phases=(0.3, 1.9)),
+    "2d2v_40x8": dict(N_q1=40, N_q2=8, N_p1=6, N_p2=8, p_dim=2, modes=((1, 2),), amps=(8e-4,),
+                      phases=(0.7,)),
+    "1d1v_64": dict(N_q1=64, N_q2=1, N_p1=16, p_dim=1, N_ghost=3, modes=((1, 0),), amps=(6e-4,),
+                    phases=(1.1,)),
+    "2d3v_off_64x4": dict(N_q1=64, N_q2=4, N_p1=6, N_p2=6, N_p3=6, p_dim=3, p_max=8.0,
+                          modes=((1, 1),), amps=(5e-3,), phases=(0.2,)),
+}
+
+
+@pytest.mark.parametrize("name", sorted(QLINE_CASES))
+@pytest.mark.parametrize("rq", ["piecewise-constant", "minmod", "ppm", "weno5"])
+@pytest.mark.parametrize("stepper", ["RK2", "SSP-RK3"])
+def test_qline_layout_per_step(gpu, name, rq, stepper):
+    """The q1-line layout kernels (every stage form, 1-3 Maxwellian factors,
+    q1 lines shorter than a wave multiple, a singleton q2) against the oracle,
+    per step from the identical state; the bar is 1e-12 or 4x the oracle's own
+    change under 1-ulp input noise, whichever is larger (WENO5's expanded
+    indicator is rounding-dominated on smooth data)."""
+    from tests.cases import maxwellian_case
+    kw = dict(QLINE_CASES[name], recon_q=rq, recon_p=rq, riemann="lax-friedrichs",
+              stepper=stepper, tau=0.02, memory_layout="q1-lines")
+    c = maxwellian_case(**kw)
+    c.dt = 5e-4
+    nls = hip_solver(c)
+    assert nls._layout == 1, "case must run in the q1-line layout"
+    sens = _oracle_self_sensitivity(c)
+    o = oracle_solver(c)
+    worst = 0.0
+    for _ in range(2):
+        nls.f = o.f.to(nls.device)
+        nls.time_elapsed = o.time_elapsed
+        o.strang_timestep(c.dt)
+        nls.strang_timestep(c.dt)
+        worst = max(worst, _cmp(nls, o)[0])
+    print(f"{name} {rq} {stepper}: per-step rel Linf {worst:.3e} (oracle sensitivity {sens:.3e})",
+          flush=True)
+    assert worst <= max(1e-12, 4 * sens)
