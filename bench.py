@@ -74,6 +74,74 @@ def _traffic(precision):
         return None
 
 
+def run_adjoint(steps=100, every=10):
+    """The adjoint workload: 2D2V 64^2 x 32^2, WENO5 + LLF + BGK tau = 0.05,
+    SSP-RK3 dt = 5e-4, 100 steps with checkpoints every 10.  Times the
+    forward run and the backward run (segment recompute + stage adjoints)."""
+    import torch
+    from bolt_amd import adjoint as A
+    from tests.cases import case_c4, hip_solver, maxwellian_interior
+    c = case_c4(seed=0)
+    s = hip_solver(c)
+    v = [x.reshape(-1) for x in (s.p1_center, s.p2_center, s.p3_center)]
+    dq = s.dq1 * s.dq2
+    dn = c.dn.to(s.device).requires_grad_(True)
+    tau = torch.tensor(0.05, dtype=torch.float64, device=s.device, requires_grad=True)
+
+    def once():
+        fT = A.evolve(s, maxwellian_interior(*v, 2, dn), tau, steps, c.dt, every)
+        n, E = A.moment(s, fT, "density"), A.moment(s, fT, "energy")
+        J = ((n - 1) ** 2).sum() * dq + E.sum() * dq
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        torch.autograd.grad(J, [dn, tau])
+        torch.cuda.synchronize()
+        return t1, time.perf_counter()
+
+    once()  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t1, t2 = once()
+    cells = s._Q * s._P
+    return {"workload": "2D2V adjoint 64^2 q x 32^2 p, WENO5 + LLF + BGK tau=0.05, SSP-RK3 "
+                        "dt=5e-4, %d steps, checkpoints every %d, FP64" % (steps, every),
+            "forward_ms": (t1 - t0) * 1e3, "backward_ms": (t2 - t1) * 1e3,
+            "cell_steps_per_s_forward_plus_backward": cells * steps / (t2 - t0)}
+
+
+def run_em(steps=10, warmup=2):
+    """2D3V electromagnetic step: 64^2 q x 24^3 p on [-8, 8]^3, electrons
+    (q/m = -1) in B = (0, 0, 0.5), PPM in q, WENO5 in p, LLF, BGK tau = 0.1,
+    SSP-RK3 dt = 2e-4, Yee FDTD fields; FP32-bulk and plain FP64, with the
+    FP32-bulk error against the FP64 run after the same steps."""
+    import torch
+    from tests.cases import case_2d3v, hip_solver
+    out = {}
+    f = {}
+    for prec in ("fp64", "fp32-bulk"):
+        c = case_2d3v(seed=0, precision=prec)
+        s = hip_solver(c)
+        for _ in range(warmup):
+            s._advance(c.dt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s._advance(c.dt)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        s.check_divergence()
+        out[prec] = {"ms_per_step": dt * 1e3, "value": s._Q * s._P / dt}
+        f[prec] = s._f.double()
+        del s
+    rel = (torch.linalg.vector_norm(f["fp32-bulk"] - f["fp64"]) /
+           torch.linalg.vector_norm(f["fp64"])).item()
+    return {"workload": "2D3V EM 64^2 q x 24^3 p, PPM q / WENO5 p, LLF, BGK tau=0.1, "
+                        "SSP-RK3 dt=2e-4, FDTD fields",
+            "cells": 64 * 64 * 24 ** 3, "fp32_bulk": out["fp32-bulk"], "fp64": out["fp64"],
+            "fp32_bulk_rel_l2_vs_fp64_after_%d_steps" % (steps + warmup): rel,
+            "speedup_fp32_bulk_vs_fp64": out["fp32-bulk"]["value"] / out["fp64"]["value"]}
+
+
 def cpu_baseline():
     """The FP64 oracle (torch CPU, all host threads) on a bounded sample of the
     same problem: 64^2 q-cells x 48^2 p-cells, one SSP-RK3 step."""
@@ -116,6 +184,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the 2D3V electromagnetic and adjoint measurements")
     ap.add_argument("--probe", choices=["fp64", "fp32-bulk"],
                     help="traffic probe for rocprofv3 --pmc: a 256 MiB device copy "
                          "(the counter calibration) and then one step of the workload")
@@ -158,6 +228,9 @@ def main():
                       "rel_l2_vs_fp64_after_run": rel_l2,
                       "speedup_vs_fp64": r32["value"] / r64["value"]},
     }
+    if not args.no_extras:
+        out["em_2d3v"] = run_em()
+        out["adjoint"] = run_adjoint()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

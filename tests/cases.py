@@ -153,3 +153,43 @@ def hip_solver(case):
     system = physical_system(case.domain, case.boundary_conditions, case.params,
                              case.initialize, adv, coll.BGK, mom)
     return nonlinear_solver(system)
+
+
+def smoothed_noise(seed, N1, N2, sigma=2.0, amp=1e-3):
+    """Gaussian-smoothed N(0, 1) noise on the periodic N1 x N2 grid (FFT
+    convolution, sigma in cells), rescaled to max |dn| = amp and stored in
+    FP32, returned as float64."""
+    rng = np.random.default_rng(seed)
+    x = torch.from_numpy(rng.standard_normal((N1, N2)))
+    k1 = torch.fft.fftfreq(N1).reshape(-1, 1)
+    k2 = torch.fft.fftfreq(N2).reshape(1, -1)
+    g = torch.exp(-2 * (math.pi * sigma) ** 2 * (k1 ** 2 + k2 ** 2))
+    y = torch.fft.ifft2(torch.fft.fft2(x) * g).real
+    y = y * (amp / y.abs().max())
+    return y.float().double()
+
+
+def case_c4(seed=0, N_q=64, N_p=32, **over):
+    """Adjoint workload: 2D2V fields off, WENO5 + LLF + BGK tau = 0.05,
+    SSP-RK3, dt = 5e-4, 100 steps, p in [-10, 10]^2; the initial state is
+    (1 + dn(q)) times the unit Maxwellian with dn = smoothed_noise."""
+    kw = dict(N_q1=N_q, N_q2=N_q, N_p1=N_p, N_p2=N_p, N_ghost=3, p_max=10.0, p_dim=2,
+              modes=((1, 1),), amps=(0.0,), phases=(0.0,), recon_q="weno5",
+              recon_p="weno5", riemann="lax-friedrichs", stepper="SSP-RK3", tau=0.05)
+    kw.update(over)
+    c = maxwellian_case(**kw)
+    c.dt, c.steps, c.checkpoint_every = 5e-4, 100, 10
+    c.dn = smoothed_noise(seed, kw["N_q1"], kw["N_q2"])
+    return c
+
+
+def maxwellian_interior(p1, p2, p3, p_dim, dn):
+    """(1 + dn(q)) * (2 pi)^(-p_dim/2) exp(-|p|^2/2) as a (P, N_q1, N_q2)
+    tensor; p1..p3 are the (P,) cell-centre velocities."""
+    e = p1 ** 2
+    if p_dim >= 2:
+        e = e + p2 ** 2
+    if p_dim >= 3:
+        e = e + p3 ** 2
+    M = (2 * math.pi) ** (-p_dim / 2) * torch.exp(-0.5 * e)
+    return M.reshape(-1, 1, 1) * (1 + dn).unsqueeze(0)
