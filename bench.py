@@ -142,6 +142,61 @@ def run_em(steps=10, warmup=2):
             "speedup_fp32_bulk_vs_fp64": out["fp32-bulk"]["value"] / out["fp64"]["value"]}
 
 
+def run_ensemble(steps=200, sample_steps=10):
+    """The ensemble workload: 96 1D2V members of uneven sizes (N_q1 in
+    {32..256}, N_p in {24..64}^2, tau in [1e-3, 1], own dt each), WENO5 + LLF
+    + BGK, SSP-RK3, FP32-bulk with FP64 moments, 200 steps in persistent
+    ensemble launches with the autotuned work-list shape; beside it the same
+    members stepped one after another by the single-problem solver in plain
+    FP64 (timed over ``sample_steps`` steps each)."""
+    import torch
+    from bolt_amd.ensemble import Ensemble
+    from tests.cases import case_c5, hip_solver
+    m32 = case_c5(seed=0, precision="fp32-bulk")
+    m64 = case_c5(seed=0, precision="fp64")
+    dts = [dt for _, dt in m32]
+    t0 = time.perf_counter()
+    ens = Ensemble([hip_solver(c) for c, _ in m32], dts)
+    torch.cuda.synchronize()
+    t_tune = time.perf_counter() - t0
+    cells = sum(s._Q * s._P for s in ens.solvers)
+    ens.step(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.step(steps)
+    torch.cuda.synchronize()
+    t_ens = time.perf_counter() - t0
+    ens.check_divergence()
+    # sequential FP64 baseline and the FP32-bulk error against it
+    solo = [hip_solver(c) for c, _ in m64]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s, (c, dt) in zip(solo, m64):
+        for _ in range(sample_steps):
+            s.strang_timestep(dt)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    chk = Ensemble([hip_solver(c) for c, _ in m32], dts, tuning=ens.tuning)
+    chk.step(sample_steps)
+    worst = 0.0
+    for i, s in enumerate(solo):
+        a, b = chk.state(i).double(), chk._lines(s)
+        worst = max(worst, float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)))
+    rate = cells * steps / t_ens
+    seq = cells * sample_steps / t_seq
+    return {"workload": "96 x 1D2V ensemble (N_q1 in {32..256}, N_p in {24..64}^2, own dt and "
+                        "tau), WENO5 + LLF + BGK, SSP-RK3, FP32-bulk, %d steps" % steps,
+            "members": len(dts), "cells": cells, "value": rate,
+            "unit": "phase-space cell-updates/s", "ms_per_step": t_ens / steps * 1e3,
+            "tuning": {"waves_per_group": ens.tuning[0], "cells_per_task": ens.tuning[1],
+                       "groups_per_cu": ens.tuning[2], "chunk_cells": ens.tuning[3]},
+            "autotune_s": t_tune,
+            "sequential_fp64": {"value": seq, "sample": "%d steps per member, one member "
+                                                        "after another" % sample_steps},
+            "speedup_vs_sequential_fp64": rate / seq,
+            "max_rel_l2_fp32_bulk_vs_fp64_after_%d_steps" % sample_steps: worst}
+
+
 def cpu_baseline():
     """The FP64 oracle (torch CPU, all host threads) on a bounded sample of the
     same problem: 64^2 q-cells x 48^2 p-cells, one SSP-RK3 step."""
@@ -231,6 +286,7 @@ def main():
     if not args.no_extras:
         out["em_2d3v"] = run_em()
         out["adjoint"] = run_adjoint()
+        out["ensemble"] = run_ensemble()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

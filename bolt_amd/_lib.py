@@ -13,10 +13,13 @@ RECON = {"piecewise-constant": 0, "minmod": 1, "ppm": 2, "weno5": 3}
 RIEMANN = {"upwind-flux": 0, "lax-friedrichs": 1}
 PRECISION = {"fp64": 0, "fp32-bulk": 1}
 LAYOUT = {"reference": 0, "q1-lines": 1}
+STEPPER = {"RK2": 0, "SSP-RK3": 1}
 
 EXPORTS = ("boltx_fvm_stage", "boltx_moments", "boltx_workspace_size", "boltx_workspace_init",
            "boltx_currents", "boltx_fdtd_step", "boltx_yee_to_cell_centered",
            "boltx_check_finite", "boltx_adjoint_stage", "boltx_adjoint_workspace_size",
+           "boltx_ensemble_layout", "boltx_ensemble_workspace_size", "boltx_ensemble_map",
+           "boltx_ensemble_init", "boltx_ensemble_step",
            "boltx_version", "boltx_last_error")
 MOM_ROWS = 12
 
@@ -56,6 +59,15 @@ class Adjoint(ctypes.Structure):
                 ("tau_grad", ctypes.c_void_p), ("workspace", ctypes.c_void_p)]
 
 
+class EnsMember(ctypes.Structure):
+    _fields_ = [("grid", Grid), ("phys", Physics), ("dt", ctypes.c_double)]
+
+
+class EnsTuning(ctypes.Structure):
+    _fields_ = [("waves_per_group", ctypes.c_int32), ("cells_per_task", ctypes.c_int32),
+                ("groups_per_cu", ctypes.c_int32), ("chunk_cells", ctypes.c_int32)]
+
+
 class BoltxError(RuntimeError):
     pass
 
@@ -86,10 +98,19 @@ def load(path=LIB_PATH):
     lib.boltx_check_finite.argtypes = [g, vp, vp, vp]
     lib.boltx_adjoint_stage.argtypes = [g, ph, ctypes.POINTER(Adjoint), vp]
     lib.boltx_adjoint_workspace_size.argtypes = [g, ph]
+    em, tu, i32 = ctypes.POINTER(EnsMember), ctypes.POINTER(EnsTuning), ctypes.c_int
+    lib.boltx_ensemble_layout.argtypes = [i32, em, ctypes.POINTER(ctypes.c_longlong)]
+    lib.boltx_ensemble_workspace_size.argtypes = [i32, em, tu]
+    lib.boltx_ensemble_map.argtypes = [i32, em, tu, ctypes.POINTER(ctypes.c_size_t),
+                                       ctypes.POINTER(ctypes.c_size_t)]
+    lib.boltx_ensemble_init.argtypes = [i32, em, tu, vp, vp, vp]
+    lib.boltx_ensemble_step.argtypes = [i32, em, tu, i32, vp, vp, vp, vp, i32, vp]
     for name in EXPORTS[:-2]:
         getattr(lib, name).restype = ctypes.c_int
     lib.boltx_workspace_size.restype = ctypes.c_size_t
     lib.boltx_adjoint_workspace_size.restype = ctypes.c_size_t
+    lib.boltx_ensemble_layout.restype = ctypes.c_longlong
+    lib.boltx_ensemble_workspace_size.restype = ctypes.c_size_t
     lib.boltx_version.argtypes = []
     lib.boltx_version.restype = ctypes.c_int
     lib.boltx_last_error.argtypes = []

@@ -12,7 +12,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 # translation units, compiled in parallel and linked into one library
-SOURCES = [os.path.join(CSRC, n) for n in ("boltx.hip", "ql_f64.hip", "ql_f32.hip")]
+SOURCES = [os.path.join(CSRC, n) for n in ("boltx.hip", "ql_f64.hip", "ql_f32.hip", "ens.hip")]
 OUT = os.path.join(PKG, "libboltx_hip.so")
 
 # -packed-fp32-ops: no v_pk_*_f32 in generated code (scalar FP32 only).

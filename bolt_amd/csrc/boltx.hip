@@ -19,13 +19,13 @@
 #include "ql_launch.h"
 #include "qline_moments.h"
 #include "adjoint.h"
+#include "bx_status.h"
 
 namespace bx {
 
 static thread_local char g_err[512];
 
-static int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
-static int fail(int code, const char* fmt, ...) {
+int fail(int code, const char* fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
   vsnprintf(g_err, sizeof(g_err), fmt, ap);
@@ -290,7 +290,7 @@ __global__ void check_finite_kernel(const double* n, int Q, int* flag) {
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
-static int check_grid(const boltx_grid* g) {
+int check_grid(const boltx_grid* g) {
   if (!g) return fail(BOLTX_EINVAL, "grid is NULL");
   if (g->n_q1 < 1 || g->n_q2 < 1 || g->n_p1 < 1 || g->n_p2 < 1 || g->n_p3 < 1)
     return fail(BOLTX_EINVAL, "grid sizes must be positive (got q %d x %d, p %d x %d x %d)",
@@ -334,7 +334,7 @@ static StageParams base_params(const boltx_grid* g) {
   return sp;
 }
 
-static int launch_status(const char* what) {
+int launch_status(const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(BOLTX_ELAUNCH, "%s: %s", what, hipGetErrorString(e));
   return BOLTX_OK;
@@ -515,7 +515,7 @@ static int ql_stage(const boltx_grid* g, const boltx_physics* ph, const boltx_st
   return BOLTX_OK;
 }
 
-static int check_physics(const boltx_physics* ph) {
+int check_physics(const boltx_physics* ph) {
   if (!ph) return fail(BOLTX_EINVAL, "physics is NULL");
   if (ph->recon_q < 0 || ph->recon_q > 3 || ph->recon_p < 0 || ph->recon_p > 3)
     return fail(BOLTX_EINVAL, "reconstruction out of range (q %d, p %d)", ph->recon_q, ph->recon_p);

@@ -1,0 +1,190 @@
+"""Ensembles of independent one-dimensional-in-q problems advanced together.
+
+The reference steps one problem per process (nonlinear_solver.py:67,
+timestep.py:101-160); many small problems then cost one launch sequence each.
+``Ensemble`` takes already built ``nonlinear_solver`` objects of uneven sizes
+(periodic q1, N_q2 = 1, fields off, the same reconstruction and precision)
+with a time step each, pools their states in the q1-line layout and advances
+all of them with ``boltx_ensemble_step``: per RK stage one persistent launch
+over a host-balanced work list, one batched moment launch and one batched
+moment-block / Maxwellian-table launch.
+
+The work-list shape (waves per workgroup, cells per wave task, persistent
+workgroups per CU, cells per moment workgroup) is picked by ``autotune``,
+which times candidate shapes on a scratch copy of the ensemble and caches the
+winner per ensemble signature.
+"""
+import ctypes
+import itertools
+import time
+
+import torch
+
+from . import _lib
+
+F64 = torch.float64
+
+DEFAULT_TUNING = (4, 1024, 4, 16384)
+_TUNED = {}
+
+
+def _tuning(t):
+    return _lib.EnsTuning(*[int(x) for x in t])
+
+
+class Ensemble:
+    def __init__(self, solvers, dts, tuning=None, autotune=True):
+        if len(solvers) != len(dts) or not solvers:
+            raise ValueError('one time step per solver')
+        self.solvers = list(solvers)
+        s0 = self.solvers[0]
+        self._L = _lib.load()
+        for s in self.solvers:
+            if s.N_q2 != 1 or s._fields_on:
+                raise NotImplementedError('ensemble members need N_q2 = 1 and fields off')
+            if s._stepper != s0._stepper or s._precision != s0._precision \
+                    or s._recon_q != s0._recon_q:
+                raise NotImplementedError('ensemble members must share stepper, precision '
+                                          'and reconstruction')
+        self.device = s0.device
+        self.dtype = s0._dtype
+        self.stepper = _lib.STEPPER[s0._stepper]
+        n = len(self.solvers)
+        self._n = n
+        self._members = (_lib.EnsMember * n)()
+        for i, (s, dt) in enumerate(zip(self.solvers, dts)):
+            self._members[i].grid = s._grid
+            self._members[i].phys = s._phys
+            self._members[i].dt = float(dt)
+        offs = (ctypes.c_longlong * n)()
+        total = self._L.boltx_ensemble_layout(n, self._members, offs)
+        if total < 0:
+            _lib.check(-1)
+        self.offsets = [int(o) for o in offs]
+        self.f = torch.empty(int(total), dtype=self.dtype, device=self.device)
+        for i, s in enumerate(self.solvers):
+            self._view(self.f, i).copy_(self._lines(s))
+        self.u1 = torch.empty_like(self.f)
+        self.u2 = torch.empty_like(self.f)
+        self.time_elapsed = 0.0
+        self.steps_taken = 0
+        key = self.signature()
+        if tuning is None:
+            tuning = _TUNED.get(key)
+        if tuning is None:
+            tuning = self.autotune() if autotune else DEFAULT_TUNING
+        self._set_tuning(tuning)
+
+    # -- layout --------------------------------------------------------------
+    def _view(self, pooled, i):
+        s = self.solvers[i]
+        P = s._P
+        return pooled[self.offsets[i]:self.offsets[i] + P * s.N_q1].view(P, s.N_q1)
+
+    @staticmethod
+    def _lines(s):
+        """A solver's interior state as [P][N_q1] (q1-line order)."""
+        f = s._f
+        if s._layout == _lib.LAYOUT['q1-lines']:
+            return f.reshape(s._P, s.N_q1)
+        return f.reshape(s.N_q1, s._P).t()
+
+    def signature(self):
+        return tuple((s.N_q1, s.N_p1, s.N_p2, s.N_p3) for s in self.solvers) + \
+            (self.stepper, int(self._members[0].phys.precision), int(self._members[0].phys.recon_q))
+
+    def _set_tuning(self, tuning):
+        self.tuning = tuple(int(x) for x in tuning)
+        self._tune = _tuning(self.tuning)
+        nws = int(self._L.boltx_ensemble_workspace_size(self._n, self._members,
+                                                        ctypes.byref(self._tune)))
+        if nws == 0:
+            _lib.check(-1)
+        self._ws = torch.empty(nws, dtype=torch.uint8, device=self.device)
+        mo, fo = ctypes.c_size_t(), ctypes.c_size_t()
+        _lib.check(self._L.boltx_ensemble_map(self._n, self._members, ctypes.byref(self._tune),
+                                              ctypes.byref(mo), ctypes.byref(fo)))
+        self._mom_off, self._flag_off = mo.value, fo.value
+        _lib.check(self._L.boltx_ensemble_init(self._n, self._members, ctypes.byref(self._tune),
+                                               _lib.ptr(self.f), _lib.ptr(self._ws),
+                                               _lib.stream_handle()))
+
+    # -- stepping ------------------------------------------------------------
+    def step(self, n_steps=1):
+        """Advance every member by n_steps of its own dt (no host sync)."""
+        _lib.check(self._L.boltx_ensemble_step(
+            self._n, self._members, ctypes.byref(self._tune), self.stepper, _lib.ptr(self.f),
+            _lib.ptr(self.u1), _lib.ptr(self.u2), _lib.ptr(self._ws), int(n_steps),
+            _lib.stream_handle()))
+        self.steps_taken += n_steps
+
+    def check_divergence(self):
+        """timestep.py:28-38 for every member (one host sync)."""
+        flag = self._ws[self._flag_off:self._flag_off + 4].view(torch.int32)
+        if int(flag.item()) != 0:
+            raise SystemExit('Solver Diverging!')
+
+    # -- results -------------------------------------------------------------
+    def state(self, i):
+        """Member i's interior f as (P, N_q1) in the storage dtype (a view)."""
+        return self._view(self.f, i)
+
+    def moments(self, i):
+        """Member i's moment block [12][N_q1] (FP64 view): density, the three
+        bulk momenta and energy (times dp1 dp2 dp3), then the BGK parameters."""
+        off = self._mom_off // 8 + sum(_lib.MOM_ROWS * s.N_q1 for s in self.solvers[:i])
+        n1 = self.solvers[i].N_q1
+        flat = self._ws[:(self._ws.numel() // 8) * 8].view(F64)
+        return flat[off:off + _lib.MOM_ROWS * n1].view(_lib.MOM_ROWS, n1)
+
+    def write_back(self):
+        """Copy every member's state into its solver (solver.f then reads it)."""
+        for i, s in enumerate(self.solvers):
+            lines = self.state(i)
+            if s._layout == _lib.LAYOUT['q1-lines']:
+                s._f.copy_(lines.reshape(s._f.shape))
+            else:
+                s._f.copy_(lines.t().reshape(s._f.shape))
+            s._update_moments()
+
+    # -- autotuner -----------------------------------------------------------
+    def autotune(self, candidates=None, steps=2):
+        """Time each candidate work-list shape over ``steps`` steps on a scratch
+        copy of the ensemble and keep the fastest; the result is cached for
+        ensembles with the same signature.  Returns the chosen tuning."""
+        if candidates is None:
+            candidates = list(itertools.product((1, 2, 4), (256, 1024, 4096), (2, 8), (16384,)))
+        saved_f = self.f
+        best, best_t = None, float('inf')
+        for cand in candidates:
+            self.f = saved_f.clone()
+            try:
+                self._set_tuning(cand)
+            except _lib.BoltxError:
+                continue
+            self.step(1)  # warm-up
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            self.step(steps)
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+            if t < best_t:
+                best, best_t = cand, t
+        # refine the moment chunking with the best shape
+        for chunk in (4096, 65536):
+            cand = best[:3] + (chunk,)
+            self.f = saved_f.clone()
+            self._set_tuning(cand)
+            self.step(1)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            self.step(steps)
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+            if t < best_t:
+                best, best_t = cand, t
+        self.f = saved_f
+        self.steps_taken = 0
+        _TUNED[self.signature()] = best
+        self.tune_time_per_step = best_t / steps
+        return best

@@ -165,6 +165,56 @@ int boltx_adjoint_stage(const boltx_grid *grid, const boltx_physics *phys,
                         const boltx_adjoint *adj, void *stream);
 size_t boltx_adjoint_workspace_size(const boltx_grid *grid, const boltx_physics *phys);
 
+/* Ensembles: independent members that are one-dimensional in q (periodic
+ * q1, n_q2 = 1, fields off, n_q1 <= 256), each with its own grids, physics
+ * and time step, advanced together.  Every member's f lives in the q1-line
+ * layout [P][n_q1] inside pooled device arrays (member i at the element
+ * offset boltx_ensemble_layout() reports); all members share the
+ * reconstruction and the precision.  Each stage of the whole ensemble is one
+ * persistent launch over a host-built work list (wave tasks of about
+ * cells_per_task cells, contiguous ranges of equal cost for
+ * groups_per_cu * #CU workgroups of waves_per_group waves), then one batched
+ * moment launch and one batched moment-block / Maxwellian-table launch.  The
+ * reference runs such ensembles one instance at a time: this replaces that
+ * loop over timestep_fvm (fvm_operator.py:6-50) for many small problems. */
+enum boltx_stepper { BOLTX_RK2 = 0, BOLTX_SSP_RK3 = 1 };
+
+typedef struct boltx_ens_member {
+  boltx_grid grid;
+  boltx_physics phys;
+  double dt;
+} boltx_ens_member;
+
+typedef struct boltx_ens_tuning {
+  int32_t waves_per_group; /* 1, 2 or 4 */
+  int32_t cells_per_task;  /* target phase-space cells per wave task */
+  int32_t groups_per_cu;   /* persistent workgroups per compute unit, 1..64 */
+  int32_t chunk_cells;     /* target cells per moment workgroup */
+} boltx_ens_tuning;
+
+/* Element offsets of the members in the pooled f-sized arrays (offsets[n],
+ * may be NULL); returns the pooled element count, or -1. */
+long long boltx_ensemble_layout(int n, const boltx_ens_member *members, long long *offsets);
+/* Bytes of device workspace (work lists, moment blocks, partial moments,
+ * Maxwellian tables, divergence flag) for this ensemble and tuning; 0 on
+ * invalid input. */
+size_t boltx_ensemble_workspace_size(int n, const boltx_ens_member *members,
+                                     const boltx_ens_tuning *tuning);
+/* Byte offsets inside the workspace of the moment blocks (member i's
+ * [BOLTX_MOM_ROWS][n_q1] block after those of members 0..i-1) and of the
+ * int32 divergence flag. */
+int boltx_ensemble_map(int n, const boltx_ens_member *members, const boltx_ens_tuning *tuning,
+                       size_t *moments_offset, size_t *flag_offset);
+/* Writes the work lists into the workspace (synchronising the stream once)
+ * and computes the moment blocks and tables of the pooled state f. */
+int boltx_ensemble_init(int n, const boltx_ens_member *members, const boltx_ens_tuning *tuning,
+                        const void *f, void *workspace, void *stream);
+/* n_steps steps of every member (stepper: enum boltx_stepper), f in place,
+ * u1 and u2 pooled scratch states (u2 unused by RK2). */
+int boltx_ensemble_step(int n, const boltx_ens_member *members, const boltx_ens_tuning *tuning,
+                        int stepper, void *f, void *u1, void *u2, void *workspace, int n_steps,
+                        void *stream);
+
 int boltx_version(void);
 const char *boltx_last_error(void);
 

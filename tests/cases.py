@@ -193,3 +193,31 @@ def maxwellian_interior(p1, p2, p3, p_dim, dn):
         e = e + p3 ** 2
     M = (2 * math.pi) ** (-p_dim / 2) * torch.exp(-0.5 * e)
     return M.reshape(-1, 1, 1) * (1 + dn).unsqueeze(0)
+
+
+def case_c5(seed=0, n=96, precision="fp32-bulk", **over):
+    """Ensemble workload: n independent 1D2V members (periodic q1 in [0, 1),
+    N_q2 = 1, p in [-10, 10]^2), each drawn from the seed: N_q1 in
+    {32, 64, 128, 256}, N_p1 = N_p2 in {24, 32, 48, 64}, tau log-uniform in
+    [1e-3, 1], density mode k in {1..4} with A ~ U[1e-4, 1e-2]; Maxwellian
+    T = 1, WENO5 + LLF + BGK, SSP-RK3 at dt = 0.4 dq1 / max|v1|.  Returns a
+    list of (case, dt)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        nq = int(rng.choice([32, 64, 128, 256]))
+        npp = int(rng.choice([24, 32, 48, 64]))
+        tau = float(10 ** rng.uniform(-3, 0))
+        k = int(rng.integers(1, 5))
+        A = float(rng.uniform(1e-4, 1e-2))
+        phi = float(rng.uniform(0, 2 * math.pi))
+        kw = dict(N_q1=nq, N_q2=1, N_p1=npp, N_p2=npp, N_ghost=3, p_max=10.0, p_dim=2,
+                  modes=((k, 0),), amps=(A,), phases=(phi,), recon_q="weno5", recon_p="weno5",
+                  riemann="lax-friedrichs", stepper="SSP-RK3", tau=tau, precision=precision)
+        kw.update(over)
+        c = maxwellian_case(**kw)
+        vmax = 10.0 - 0.5 * 20.0 / kw["N_p1"]
+        dt = 0.4 * (1.0 / kw["N_q1"]) / vmax
+        c.dt, c.steps = dt, 200
+        out.append((c, dt))
+    return out
