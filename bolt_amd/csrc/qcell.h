@@ -86,6 +86,27 @@ __host__ __device__ inline T upwind_flux_diff(const T a[7], T C, bool pos) {
   }
 }
 
+// Flux difference along a p axis of the q-cell's p-block ``blk`` (stride
+// ``stride``, n cells, periodic), from the upwind states only.  C_p1 =
+// (q/m)(E1 + v2 B3 - v3 B2) does not depend on v1 (advection_terms.py:147-188),
+// and likewise for p2 and p3, so C is the same on both sides of every face
+// along its own axis and LLF = upwind for this linear flux.
+template <int R, typename T>
+__device__ inline T p_upwind_flux_diff(const T* blk, int p, int j, int n, int stride, T C) {
+  T a[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    int jj = j + k - 3;
+    if (n >= 4) {
+      jj = jj < 0 ? jj + n : (jj >= n ? jj - n : jj);
+    } else {
+      jj = ((jj % n) + n) % n;
+    }
+    a[k] = blk[p + (jj - j) * stride];
+  }
+  return upwind_flux_diff<R, T>(a, C, C > T(0));
+}
+
 struct QCParams {
   StageParams sp;
   const int* perm;    // [nslab*64] p of each slab lane, -1 = empty lane
@@ -93,10 +114,17 @@ struct QCParams {
   int nslab;
 };
 
+// Workgroup size: with p-advection the whole p-block of the q-cell sits in
+// LDS (up to ~110 KB), so one workgroup per CU is all LDS allows and it runs
+// 16 waves; without it the workgroup is 4 waves.
+template <bool PADV>
+constexpr int qc_block() { return PADV ? 1024 : 256; }
+
 template <typename T, int RQ, int RP, bool PADV>
-__global__ __launch_bounds__(256) void qcell_kernel(QCParams qp) {
+__global__ __launch_bounds__(qc_block<PADV>()) void qcell_kernel(QCParams qp) {
+  constexpr int NW = qc_block<PADV>() / 64;
   extern __shared__ unsigned char smem[];
-  __shared__ double red[5][4];
+  __shared__ double red[5][NW];
   const StageParams& sp = qp.sp;
   T* tab = reinterpret_cast<T*>(smem);
   T* blk = tab + (sp.n_p1 + sp.n_p2 + sp.n_p3);
@@ -130,7 +158,7 @@ __global__ __launch_bounds__(256) void qcell_kernel(QCParams qp) {
   }
   __syncthreads();
 
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const T* wv = (const T*)sp.w;
   const T* unv = (const T*)sp.un;
   T* outv = (T*)sp.out;
@@ -147,7 +175,7 @@ __global__ __launch_bounds__(256) void qcell_kernel(QCParams qp) {
     nb2[k] = (long)c.nb2[k] * P;
   }
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int s = wave; s < qp.nslab; s += 4) {
+  for (int s = wave; s < qp.nslab; s += NW) {
     const int p_lane = qp.perm[s * 64 + lane];
     const bool valid = p_lane >= 0;
     const int p = valid ? p_lane : 0;
@@ -184,17 +212,16 @@ __global__ __launch_bounds__(256) void qcell_kernel(QCParams qp) {
       T dp = T(0);
       if (sp.n_p1 > 1) {
         const double C = em * (c.E[0] + v2 * c.B[0][2] - v3 * c.B[0][1]);
-        dp = p_flux_diff<RP, T>(blk, p, j1, sp.n_p1, 1, T(C), sp.riemann_p) * T(sp.inv_dp1);
+        dp = p_upwind_flux_diff<RP, T>(blk, p, j1, sp.n_p1, 1, T(C)) * T(sp.inv_dp1);
       }
       if (sp.n_p2 > 1) {
         const double C = em * (c.E[1] + v3 * c.B[1][0] - v1 * c.B[1][2]);
-        dp = dp + p_flux_diff<RP, T>(blk, p, j2, sp.n_p2, sp.n_p1, T(C), sp.riemann_p) *
-                      T(sp.inv_dp2);
+        dp = dp + p_upwind_flux_diff<RP, T>(blk, p, j2, sp.n_p2, sp.n_p1, T(C)) * T(sp.inv_dp2);
       }
       if (sp.n_p3 > 1) {
         const double C = em * (c.E[2] + v1 * c.B[2][1] - v2 * c.B[2][0]);
-        dp = dp + p_flux_diff<RP, T>(blk, p, j3, sp.n_p3, sp.n_p1 * sp.n_p2, T(C),
-                                     sp.riemann_p) * T(sp.inv_dp3);
+        dp = dp + p_upwind_flux_diff<RP, T>(blk, p, j3, sp.n_p3, sp.n_p1 * sp.n_p2, T(C)) *
+                  T(sp.inv_dp3);
       }
       d = d - dp;
     }
@@ -224,7 +251,11 @@ __global__ __launch_bounds__(256) void qcell_kernel(QCParams qp) {
     __syncthreads();
     if (threadIdx.x == 0) {
       double m[5];
-      for (int k = 0; k < 5; ++k) m[k] = ((red[k][0] + red[k][1]) + red[k][2] + red[k][3]) * sp.measure;
+      for (int k = 0; k < 5; ++k) {
+        double x = red[k][0];
+        for (int w = 1; w < NW; ++w) x += red[k][w];
+        m[k] = x * sp.measure;
+      }
       write_moment_block(sp.mom_out, sp.Q, q, m, sp.mass, sp.kB, sp.p_dim);
     }
   }
