@@ -11,6 +11,7 @@ change all of them).  The FDTD update and the Yee-to-centre averaging run in
 the HIP library.
 """
 import ctypes
+import math
 
 import torch
 
@@ -20,7 +21,8 @@ F64 = torch.float64
 
 
 class fields_solver(object):
-    """fields.py:19-690 for fields_type 'electrodynamic', fields_solver 'fdtd'."""
+    """fields.py:19-690 for fields_type 'electrodynamic' with fields_solver
+    'fdtd', and fields_type 'electrostatic' with fields_solver 'fft'."""
 
     def __init__(self, solver):
         self._solver = solver
@@ -33,12 +35,23 @@ class fields_solver(object):
         dev = solver.device
         self._yee = torch.zeros(6, Q, dtype=F64, device=dev)
         cc = torch.zeros(6, Q, dtype=F64, device=dev)
-        if p.fields_initialize != 'user-defined':
-            raise NotImplementedError(
-                "fields_initialize = %r: only 'user-defined' is supported" % p.fields_initialize)
-        self._initialize_electric_fields()
-        self._initialize_magnetic_fields()
-        self._yee_to_cc(cc)
+        self._fdtd = getattr(p, 'fields_solver', 'fdtd').lower() == 'fdtd'
+        init = p.fields_initialize
+        if init == 'user-defined':  # fields.py:474-477
+            self._initialize_electric_fields()
+            self._initialize_magnetic_fields()
+            self._yee_to_cc(cc)
+        elif init in ('fft', 'fft + user-defined magnetic fields'):  # fields.py:465-488
+            rho = solver._charge * solver._mom[0]
+            self._poisson(rho, cc)
+            self._cc_to_yee_E(cc)
+            if init != 'fft':
+                self._initialize_magnetic_fields()
+                tmp = torch.empty_like(cc)
+                self._yee_to_cc(tmp)
+                cc[3:] = tmp[3:]
+        else:
+            raise NotImplementedError('Method not valid/not implemented')
         # fields.py:493-494: all three names refer to one object.
         self._cc = cc
         self._cc_at_n = cc
@@ -70,6 +83,40 @@ class fields_solver(object):
         self._yee[4] = self._eval_interior(ini.initialize_B, s.q1_left_center, s.q2_left_center, 1)
         self._yee[5] = self._eval_interior(ini.initialize_B, s.q1_left_bot, s.q2_left_bot, 2)
 
+    def _poisson(self, rho, cc):
+        """electrostatic/fft.py:9-78 on the device (rocFFT through torch.fft):
+        rho ([Q], index i1 + N_q1 i2) minus its mean, phi_hat = rho_hat /
+        (4 pi^2 |k|^2) with phi_hat[0, 0] = 0, E = -grad phi / eps into rows 0
+        and 1 of cc."""
+        s = self._solver
+        N1, N2 = s.N_q1, s.N_q2
+        r = rho.reshape(N2, N1)
+        r = r - r.mean()
+        k1 = torch.fft.fftfreq(N1, float(self.dq1), dtype=F64, device=r.device).reshape(1, N1)
+        k2 = torch.fft.fftfreq(N2, float(self.dq2), dtype=F64, device=r.device).reshape(N2, 1)
+        den = 4 * math.pi ** 2 * (k1 ** 2 + k2 ** 2)
+        den[0, 0] = 1.0
+        phi = torch.fft.fft2(r) / den
+        phi[0, 0] = 0
+        eps = float(self.params.eps)
+        cc[0] = (torch.fft.ifft2(-2j * math.pi * k1 * phi).real / eps).reshape(-1)
+        cc[1] = (torch.fft.ifft2(-2j * math.pi * k2 * phi).real / eps).reshape(-1)
+
+    def compute_electrostatic_fields(self, rho):
+        """fields.py:559-566 (one species: no sum needed)."""
+        self._poisson(rho, self._cc)
+
+    def _cc_to_yee_E(self, cc):
+        """fields.py:497-525 for E: E1 at (i, j+1/2) = (E1[i] + E1[i-1]) / 2,
+        E2 at (i+1/2, j) = (E2[j] + E2[j-1]) / 2, E3 unchanged."""
+        s = self._solver
+        N1, N2 = s.N_q1, s.N_q2
+        e1 = cc[0].reshape(N2, N1)
+        e2 = cc[1].reshape(N2, N1)
+        self._yee[0] = (0.5 * (e1 + torch.roll(e1, 1, 1))).reshape(-1)
+        self._yee[1] = (0.5 * (e2 + torch.roll(e2, 1, 0))).reshape(-1)
+        self._yee[2] = cc[2]
+
     def _yee_to_cc(self, cc):
         s = self._solver
         _lib.check(s._L.boltx_yee_to_cell_centered(ctypes.byref(s._grid), _lib.ptr(self._yee),
@@ -96,7 +143,10 @@ class fields_solver(object):
     def _take_for_C_p(self):
         """The [6][Q] array one C_p evaluation sees, then the toggle of
         fields.py:685-688."""
-        src = self._cc_at_n if self.at_n else self._cc_at_n_plus_half
+        if not self._fdtd:
+            src = self._cc
+        else:
+            src = self._cc_at_n if self.at_n else self._cc_at_n_plus_half
         if self.params.solver_method_in_p == 'FVM':
             self.at_n = not self.at_n
         return src

@@ -27,6 +27,7 @@ import types
 import torch
 
 from . import _lib
+from . import file_io as _file_io
 from .fields import fields_solver as _fields_solver
 from .src.nonrelativistic_boltzmann import advection_terms as _adv
 from .src.nonrelativistic_boltzmann import collision_operator as _coll
@@ -163,11 +164,19 @@ class nonlinear_solver(object):
                 raise NotImplementedError('BGK needs the built-in moments module')
         self._fields_on = bool(getattr(p, 'fields_enabled', False))
         self._p_adv = self._fields_on and getattr(p, 'solver_method_in_p', 'FVM') == 'FVM'
+        self._es = False
         if self._fields_on:
-            if getattr(p, 'fields_type', None) != 'electrodynamic':
-                raise NotImplementedError('electrostatic fields are not provided')
-            if getattr(p, 'fields_solver', '').lower() != 'fdtd':
-                raise NotImplementedError('only the FDTD field solver is provided')
+            ftype = getattr(p, 'fields_type', None)
+            fsolver = getattr(p, 'fields_solver', '').lower()
+            if ftype == 'electrodynamic':
+                if fsolver != 'fdtd':
+                    raise NotImplementedError('electrodynamic fields need the FDTD solver')
+            elif ftype == 'electrostatic':
+                if fsolver != 'fft':
+                    raise NotImplementedError('electrostatic fields need the FFT solver')
+                self._es = True
+            else:
+                raise NotImplementedError('fields_type %r is not provided' % ftype)
             if getattr(p, 'hybrid_model_enabled', False):
                 raise NotImplementedError('the hybrid model is not provided')
             if self._p_adv and ps.C_p is not _adv.C_p:
@@ -290,7 +299,9 @@ class nonlinear_solver(object):
         F = [None, None, None]
         if self._p_adv:
             fs = self.fields_solver
-            if fs.at_n is False:  # df_dt_fvm.py:102-104, 214-233
+            if self._es:  # df_dt_fvm.py:235-242: rho of the stage input
+                fs.compute_electrostatic_fields(self._charge * mom_in[0])
+            elif fs.at_n is False:  # df_dt_fvm.py:102-104, 214-233
                 J = self._buf('J', F64, (3, self._Q))
                 _lib.check(self._L.boltx_currents(ctypes.byref(self._grid),
                                                   ctypes.byref(self._phys), _lib.ptr(u),
@@ -367,13 +378,32 @@ class nonlinear_solver(object):
         """Moments shaped (1, N_s, Q1g, Q2g).  The built-in five come from the
         HIP reduction; other moment functions of the user's module are
         evaluated with torch on the device."""
-        if f is None and moment_name in _mom.BUILTIN and \
+        if f is None and moment_name in _mom._BUILTIN and \
                 self.physical_system.moments is _mom:
-            return self._q_padded(self._mom[_mom.BUILTIN.index(moment_name)])
+            return self._q_padded(self._mom[_mom._BUILTIN.index(moment_name)])
         fn = getattr(self.physical_system.moments, moment_name)
         ff = self.f if f is None else torch.as_tensor(f, device=self.device)
         return fn(ff.to(F64), self.p1_center, self.p2_center, self.p3_center,
                   self.dp3 * self.dp2 * self.dp1)
+
+    # -- checkpoints (nonlinear_solver.py:513-519, file_io/dump.py, load.py) ---
+    def dump_distribution_function(self, file_name):
+        _file_io.dump_distribution_function(self, file_name)
+
+    def dump_moments(self, file_name):
+        _file_io.dump_moments(self, file_name)
+
+    def dump_EM_fields(self, file_name):
+        _file_io.dump_EM_fields(self, file_name)
+
+    def dump_aux_arrays(self, arrays, name, file_name):
+        _file_io.dump_aux_arrays(self, arrays, name, file_name)
+
+    def load_distribution_function(self, file_name):
+        _file_io.load_distribution_function(self, file_name)
+
+    def load_EM_fields(self, file_name):
+        _file_io.load_EM_fields(self, file_name)
 
     def print_performance_timings(self, N_iters=None):
         """performance_timings.py:9-165 (reduced: one GPU, one table row)."""

@@ -55,4 +55,4 @@ def q_q3(f, v1, v2, v3, integral_measure):
 
 
 # The moments the fused stage kernel produces, in its output order.
-BUILTIN = ("density", "mom_v1_bulk", "mom_v2_bulk", "mom_v3_bulk", "energy")
+_BUILTIN = ("density", "mom_v1_bulk", "mom_v2_bulk", "mom_v3_bulk", "energy")

@@ -411,6 +411,26 @@ def BGK(f, t, q1, q2, v1, v2, v3, moments_fn, params, flag=False):
 # ---------------------------------------------------------------------------
 # Fields: nonlinear/fields/fields.py + electrodynamic/fdtd_explicit.py
 # ---------------------------------------------------------------------------
+def fft_poisson(rho_int, dq1, dq2, eps):
+    """electrostatic/fft.py:9-78 on the interior charge density (N_q1, N_q2):
+    remove the mean, phi_hat = rho_hat / (4 pi^2 |k|^2) with k from
+    fftfreq(N, dq) (phi_hat[0, 0] = 0), E = -grad phi spectrally, divided by
+    eps.  Returns (E1, E2) shaped (N_q1, N_q2)."""
+    N1, N2 = rho_int.shape
+    k1 = torch.fft.fftfreq(N1, dq1, dtype=F64)
+    k2 = torch.fft.fftfreq(N2, dq2, dtype=F64)
+    k1, k2 = torch.meshgrid(k1, k2, indexing="ij")
+    rho = rho_int - rho_int.mean()
+    rho_hat = torch.fft.fft2(rho)
+    den = 4 * math.pi ** 2 * (k1 ** 2 + k2 ** 2)
+    den[0, 0] = 1.0
+    phi_hat = rho_hat / den
+    phi_hat[0, 0] = 0
+    E1 = torch.fft.ifft2(-1j * 2 * math.pi * k1 * phi_hat).real
+    E2 = torch.fft.ifft2(-1j * 2 * math.pi * k2 * phi_hat).real
+    return E1 / eps, E2 / eps
+
+
 class OracleFields:
     """fields.py:19-690 for the electrodynamic (FDTD) path with periodic q.
 
@@ -442,8 +462,16 @@ class OracleFields:
             self.initialize_electric_fields()
             self.initialize_magnetic_fields()
             self.yee_grid_to_cell_centered_grid()
+        elif p.fields_initialize == "fft":  # fields.py:465-472
+            self.compute_electrostatic_fields(rho_initial)
+            self.cell_centered_grid_to_yee_grid("E")
+        elif p.fields_initialize == "fft + user-defined magnetic fields":  # fields.py:479-488
+            self.compute_electrostatic_fields(rho_initial)
+            self.cell_centered_grid_to_yee_grid("E")
+            self.initialize_magnetic_fields()
+            self.yee_grid_to_cell_centered_grid("B")
         else:
-            raise NotImplementedError("only user-defined field initialisation is restated")
+            raise NotImplementedError("Method not valid/not implemented")
         # fields.py:493-494: Python aliasing, not copies.
         self.cell_centered_EM_fields_at_n = self.cell_centered_EM_fields
         self.cell_centered_EM_fields_at_n_plus_half = self.cell_centered_EM_fields
@@ -473,18 +501,48 @@ class OracleFields:
         self.yee_grid_EM_fields[4] = torch.broadcast_to(B2, (1,) + sh)[0]
         self.yee_grid_EM_fields[5] = torch.broadcast_to(B3, (1,) + sh)[0]
 
-    def yee_grid_to_cell_centered_grid(self):
-        """fields.py:527-557."""
+    def yee_grid_to_cell_centered_grid(self, which=None):
+        """fields.py:527-557 (``which`` = 'E', 'B' or both)."""
         y = self.yee_grid_EM_fields.clone()
         E1, E2, E3, B1, B2, B3 = (y[i:i + 1] for i in range(6))
         cc = self.cell_centered_EM_fields
-        cc[0] = (0.5 * (E1 + af_shift(E1, 0, 0, -1, 0)))[0]
-        cc[1] = (0.5 * (E2 + af_shift(E2, 0, 0, 0, -1)))[0]
-        cc[2] = E3[0]
-        cc[3] = (0.5 * (B1 + af_shift(B1, 0, 0, 0, -1)))[0]
-        cc[4] = (0.5 * (B2 + af_shift(B2, 0, 0, -1, 0)))[0]
-        cc[5] = (0.25 * (B3 + af_shift(B3, 0, 0, 0, -1) + af_shift(B3, 0, 0, -1, 0)
-                         + af_shift(B3, 0, 0, -1, -1)))[0]
+        if which in ("E", None):
+            cc[0] = (0.5 * (E1 + af_shift(E1, 0, 0, -1, 0)))[0]
+            cc[1] = (0.5 * (E2 + af_shift(E2, 0, 0, 0, -1)))[0]
+            cc[2] = E3[0]
+        if which in ("B", None):
+            cc[3] = (0.5 * (B1 + af_shift(B1, 0, 0, 0, -1)))[0]
+            cc[4] = (0.5 * (B2 + af_shift(B2, 0, 0, -1, 0)))[0]
+            cc[5] = (0.25 * (B3 + af_shift(B3, 0, 0, 0, -1) + af_shift(B3, 0, 0, -1, 0)
+                             + af_shift(B3, 0, 0, -1, -1)))[0]
+
+    def cell_centered_grid_to_yee_grid(self, which=None):
+        """fields.py:497-525 (``which`` = 'E', 'B' or both)."""
+        c = self.cell_centered_EM_fields.clone()
+        E1, E2, E3, B1, B2, B3 = (c[i:i + 1] for i in range(6))
+        y = self.yee_grid_EM_fields
+        if which in ("E", None):
+            y[0] = (0.5 * (E1 + af_shift(E1, 0, 0, 1, 0)))[0]
+            y[1] = (0.5 * (E2 + af_shift(E2, 0, 0, 0, 1)))[0]
+            y[2] = E3[0]
+        if which in ("B", None):
+            y[3] = (0.5 * (B1 + af_shift(B1, 0, 0, 0, 1)))[0]
+            y[4] = (0.5 * (B2 + af_shift(B2, 0, 0, 1, 0)))[0]
+            y[5] = (0.25 * (B3 + af_shift(B3, 0, 0, 1, 0) + af_shift(B3, 0, 0, 0, 1)
+                            + af_shift(B3, 0, 0, 1, 1)))[0]
+
+    def compute_electrostatic_fields(self, rho):
+        """fields.py:559-566 + fft.py:9-78: sum rho over species, solve on the
+        interior, write E1, E2 of the cell-centred fields and refill ghosts
+        (communicate_fields + periodic apply_bcs_fields)."""
+        rho = rho.sum(dim=1, keepdim=True)
+        g1, g2 = self.N_g1, self.N_g2
+        rho_int = rho[0, 0, g1:rho.shape[2] - g1, g2:rho.shape[3] - g2]
+        E1, E2 = fft_poisson(rho_int, self.dq1, self.dq2, self.params.eps)
+        cc = self.cell_centered_EM_fields
+        cc[0, 0, g1:cc.shape[2] - g1, g2:cc.shape[3] - g2] = E1
+        cc[1, 0, g1:cc.shape[2] - g1, g2:cc.shape[3] - g2] = E2
+        cc.copy_(self._bc(cc))
 
     def fdtd_evolve_E(self, dt):
         """fdtd_explicit.py:9-82."""
@@ -546,8 +604,11 @@ class OracleFields:
     def get_fields(self):
         """fields.py:649-690 (FDTD branch; toggles ``at_n`` on every call when
         the p-solver is FVM)."""
-        src = (self.cell_centered_EM_fields_at_n if self.at_n
-               else self.cell_centered_EM_fields_at_n_plus_half)
+        if self.params.fields_solver != "fdtd":
+            src = self.cell_centered_EM_fields
+        else:
+            src = (self.cell_centered_EM_fields_at_n if self.at_n
+                   else self.cell_centered_EM_fields_at_n_plus_half)
         out = tuple(src[i:i + 1].clone() for i in range(6))
         if self.params.solver_method_in_p == "FVM":
             self.at_n = not self.at_n
@@ -701,8 +762,11 @@ class OracleSolver:
                 J2 = self.charge * self.compute_moments("mom_v2_bulk", f=f_bot)
                 J3 = self.charge * self.compute_moments("mom_v3_bulk", f=f)
                 fs.evolve_electrodynamic_fields(J1, J2, J3, self.dt)
-            if p.fields_type == "electrostatic":
-                raise NotImplementedError("electrostatic coupling is not restated")
+            if p.fields_type == "electrostatic":  # df_dt_fvm.py:235-242
+                if p.fields_solver != "fft":
+                    raise NotImplementedError("only the FFT electrostatic solver is restated")
+                rho = self.charge * self.compute_moments("density", f=f)
+                fs.compute_electrostatic_fields(rho)
             C_p1 = as4(self._C_p(self.time_elapsed, self.q1_center, self.q2_center,
                                  self.p1_left, self.p2_left, self.p3_left, fs, p)[0])
             C_p2 = as4(self._C_p(self.time_elapsed, self.q1_center, self.q2_center,

@@ -31,7 +31,8 @@ def maxwellian_case(*, N_q1, N_q2, N_p1, N_p2=1, N_p3=1, N_ghost=3, p_max=10.0,
                     recon_q="weno5", recon_p="weno5", riemann="upwind-flux",
                     stepper="SSP-RK3", tau=0.01, source=True, fields=False,
                     B0=0.5, charge=-1.0, mass=1.0, precision="fp64",
-                    q2_bc="periodic", p3_max=None, memory_layout="auto"):
+                    q2_bc="periodic", p3_max=None, memory_layout="auto",
+                    fields_type="electrodynamic", fields_initialize="user-defined"):
     """Maxwellian (T = 1, u = 0) with density n = 1 + sum A_k cos(2 pi (kx q1 +
     ky q2) + phi_k) on [0,1)^2, p in [-p_max, p_max] per active axis (singleton
     axes span [-0.5, 0.5] as in the reference examples)."""
@@ -48,8 +49,9 @@ def maxwellian_case(*, N_q1, N_q2, N_p1, N_p2=1, N_p3=1, N_ghost=3, p_max=10.0,
     bc = _module("boundary_conditions", in_q1_left="periodic", in_q1_right="periodic",
                  in_q2_bottom=q2_bc, in_q2_top=q2_bc)
     params = _module(
-        "params", fields_type="electrodynamic", fields_initialize="user-defined",
-        fields_solver="fdtd", solver_method_in_q="FVM", solver_method_in_p="FVM",
+        "params", fields_type=fields_type, fields_initialize=fields_initialize,
+        fields_solver="fdtd" if fields_type == "electrodynamic" else "fft",
+        solver_method_in_q="FVM", solver_method_in_p="FVM",
         reconstruction_method_in_q=recon_q, reconstruction_method_in_p=recon_p,
         riemann_solver_in_q=riemann, riemann_solver_in_p=riemann, p_dim=p_dim,
         num_devices=1, mass=[mass], boltzmann_constant=1, charge=[charge],
@@ -221,3 +223,24 @@ def case_c5(seed=0, n=96, precision="fp32-bulk", **over):
         c.dt, c.steps = dt, 200
         out.append((c, dt))
     return out
+
+
+def case_es(seed=0, N_q1=32, N_q2=1, N_p=24, **over):
+    """Electrostatic (FFT Poisson) problem: electrons (q/m = -1) on a
+    neutralising background, density mode (1, 0) (and (1, 1) in 2D) with
+    A ~ U[1e-3, 1e-2], p in [-6, 6] per active axis, minmod in q and p, LLF,
+    BGK tau = 0.1, RK2 (the reference stepper), dt = 1e-3."""
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(1e-3, 1e-2)
+    phi = rng.uniform(0, 2 * math.pi)
+    two_d = N_q2 > 1
+    kw = dict(N_q1=N_q1, N_q2=N_q2, N_p1=N_p, N_p2=N_p if two_d else 1, N_ghost=3,
+              p_max=6.0, p_dim=2 if two_d else 1,
+              modes=((1, 1),) if two_d else ((1, 0),), amps=(A,), phases=(phi,),
+              recon_q="minmod", recon_p="minmod", riemann="lax-friedrichs", stepper="RK2",
+              tau=0.1, fields=True, charge=-1.0, B0=0.0, fields_type="electrostatic",
+              fields_initialize="fft")
+    kw.update(over)
+    c = maxwellian_case(**kw)
+    c.dt, c.steps = 1e-3, 100
+    return c

@@ -2,8 +2,9 @@
 
 The reference ships no golden vectors, so these are oracle outputs on small
 seeded problems: the 1D1V minmod/RK2 case after 1 and 100 steps, one step of a
-small 2D2V problem per reconstruction x stepper, and one step of a small 2D3V
-electromagnetic problem.  They pin the oracle against accidental change and
+small 2D2V problem per reconstruction x stepper, one step of a small 2D3V
+electromagnetic problem and ten steps of a 1D1V electrostatic (FFT Poisson)
+problem.  They pin the oracle against accidental change and
 give the GPU tests fixed expected outputs.  Run from the repository root:
 
     python -m tests.golden.make_golden
@@ -17,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 def golden_cases():
     """name -> (case factory, steps)."""
-    from tests.cases import case_2d2v, case_2d3v, case_c1
+    from tests.cases import case_2d2v, case_2d3v, case_c1, case_es
     out = {"c1_1step": (lambda: case_c1(seed=0), 1), "c1_100steps": (lambda: case_c1(seed=0), 100)}
     for rq in ("piecewise-constant", "minmod", "ppm", "weno5"):
         for st in ("RK2", "SSP-RK3"):
@@ -25,6 +26,7 @@ def golden_cases():
                 lambda rq=rq, st=st: _dt(case_2d2v(seed=1, N_q=8, N_p=6, recon_q=rq,
                                                    recon_p=rq, stepper=st), 1e-3), 1)
     out["2d3v_em"] = (lambda: _dt(case_2d3v(seed=2, N_q=6, N_p=4), 1e-3), 1)
+    out["1d1v_electrostatic_10steps"] = (lambda: case_es(seed=3), 10)
     return out
 
 

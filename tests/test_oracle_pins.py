@@ -421,3 +421,38 @@ def test_fdtd_pins_catch_planted_mistakes(monkeypatch):
     monkeypatch.setattr(bo.OracleFields, "fdtd_evolve_E", bad_e)
     with pytest.raises(AssertionError):
         test_fdtd_plane_wave_second_order()
+
+
+# -- electrostatic FFT Poisson --------------------------------------------------
+def _single_mode_error(k1, k2, N1=16, N2=12, eps=1.0):
+    """fft.py:9-78 on rho = 3 + A cos(2 pi (k1 x + k2 y)) at cell centres: the
+    mean is removed and the field is spectrally exact,
+    E = -grad phi with lap phi = -rho, i.e.
+    E = A (k1, k2) sin(.) / (2 pi |k|^2 eps)."""
+    L1, L2 = 1.0, 2.0
+    x = (np.arange(N1) + 0.5) * L1 / N1
+    y = (np.arange(N2) + 0.5) * L2 / N2
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    A = 0.7
+    ph = 2 * np.pi * (k1 * X / L1 + k2 * Y / L2)
+    rho = 3.0 + A * np.cos(ph)
+    E1, E2 = bo.fft_poisson(T(rho), L1 / N1, L2 / N2, eps)
+    kk1, kk2 = k1 / L1, k2 / L2
+    ksq = kk1 ** 2 + kk2 ** 2
+    w1 = A * kk1 * np.sin(ph) / (2 * np.pi * ksq * eps)
+    w2 = A * kk2 * np.sin(ph) / (2 * np.pi * ksq * eps)
+    return max(np.abs(E1.numpy() - w1).max(), np.abs(E2.numpy() - w2).max())
+
+
+@pytest.mark.parametrize("k1,k2", [(1, 0), (0, 2), (2, 1), (3, -2)])
+def test_fft_poisson_single_mode_closed_form(k1, k2):
+    assert _single_mode_error(k1, k2) <= 1e-13
+    assert _single_mode_error(k1, k2, eps=2.5) <= 1e-13
+
+
+def test_fft_poisson_catches_planted_mistakes(monkeypatch):
+    good = bo.fft_poisson
+    monkeypatch.setattr(bo, "fft_poisson", _mutant(good, "E1 = torch.fft.ifft2(-1j", "E1 = torch.fft.ifft2(1j"))
+    assert _single_mode_error(2, 1) > 1e-3
+    monkeypatch.setattr(bo, "fft_poisson", _mutant(good, "4 * math.pi ** 2", "2 * math.pi ** 2"))
+    assert _single_mode_error(2, 1) > 1e-3
