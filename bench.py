@@ -197,23 +197,46 @@ def run_ensemble(steps=200, sample_steps=10):
             "max_rel_l2_fp32_bulk_vs_fp64_after_%d_steps" % sample_steps: worst}
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline():
-    """The FP64 oracle (torch CPU, all host threads) on a bounded sample of the
-    same problem: 64^2 q-cells x 48^2 p-cells, one SSP-RK3 step."""
+    """The FP64 oracle (torch CPU eager, the reference's whole-array op
+    sequence) on bounded samples of the same problem: 64^2 q-cells x 48^2
+    p-cells with every host thread this process may use, and 32^2 x 24^2 on
+    one thread; one SSP-RK3 step each."""
     import torch
     from tests.cases import case_2d2v, oracle_solver
-    n = os.cpu_count() or 1
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+
+    def rate(threads, nq, npp):
+        torch.set_num_threads(threads)
+        c = case_2d2v(seed=0, N_q=nq, N_p=npp)
+        o = oracle_solver(c)
+        t0 = time.perf_counter()
+        o.strang_timestep(c.dt)
+        return nq * nq * npp * npp / (time.perf_counter() - t0)
+
+    v = rate(n, 64, 48)
+    v1 = rate(1, 32, 24)
     torch.set_num_threads(n)
-    c = case_2d2v(seed=0, N_q=64)
-    o = oracle_solver(c)
-    t0 = time.perf_counter()
-    o.strang_timestep(c.dt)
-    dt = time.perf_counter() - t0
-    cells = 64 * 64 * 48 * 48
-    return {"value": cells / dt, "unit": "phase-space cell-updates/s", "cores": n,
+    return {"value": v, "unit": "phase-space cell-updates/s", "cores": n,
             "kind": "port",
             "sample": "FP64 oracle (torch CPU), 2D2V WENO5/LLF/BGK SSP-RK3 at "
-                      "64^2 x 48^2 cells, 1 step (%.1f s)" % dt}
+                      "64^2 x 48^2 cells, 1 step, %d threads" % n,
+            "cpu_model": _cpu_model(),
+            "one_thread": {"value": v1, "sample": "32^2 x 24^2 cells, 1 step, 1 thread"}}
 
 
 def probe(precision):

@@ -456,7 +456,8 @@ static int ql_moments(const boltx_grid* g, const StageParams& sp, const void* f,
   }
   int rc = launch_status("mline_kernel");
   if (rc) return rc;
-  hipLaunchKernelGGL(ml_finalize_kernel, dim3((sp.Q + 255) / 256), dim3(256), 0, s, part,
+  // one wave per workgroup: Q threads spread over every CU
+  hipLaunchKernelGGL(ml_finalize_kernel, dim3((sp.Q + 63) / 64), dim3(64), 0, s, part,
                      ML_CHUNKS, sp.Q, sp.measure, mom, sp.mass, sp.kB, sp.p_dim);
   return launch_status("ml_finalize_kernel");
 }
@@ -464,6 +465,8 @@ static int ql_moments(const boltx_grid* g, const StageParams& sp, const void* f,
 template <typename T>
 static int ql_stage(const boltx_grid* g, const boltx_physics* ph, const boltx_stage* st,
                     const StageParams& sp, hipStream_t s) {
+  if (g->n_q1 > 1 && g->n_q1 < QL_HALO)
+    return fail(BOLTX_EUNSUPPORTED, "the q1-line layout needs n_q1 = 1 or n_q1 >= %d", QL_HALO);
   char* ws = (char*)st->workspace;
   T* tab = (T*)ws;
   double* part = (double*)(ws + ql_table_bytes(g, ph->precision));

@@ -55,12 +55,18 @@ struct EnsStageArgs {
   int use_un, b_one;
 };
 
+// LDS per wave: q1 lines with QL_HALO periodic copies on each side; a wave
+// packs 64 / n1 lines when n1 <= 64 (at most 12 lines of n1 + 6 cells for
+// n1 >= 5) or holds one line of up to 256 cells.
+constexpr int ENS_LDS = 264;
+
 // Stencil, source and RK combination of the q1 lines of one wave.  Lane l
-// works on cells i1 = c0 + 64 k (k < NR) of line ``line`` (p-cell p), all
-// lanes of a line sharing C1 and the table rows.
+// works on cells i1 = c0 + 64 k (k < NR) of the line starting at ``line``
+// (LDS, halo-padded; p-cell p), all lanes of a line sharing C1 and the table
+// rows.
 template <typename T, int R, int NR>
-__device__ inline void ens_lines(const EnsStageArgs& ea, const EnsInst& I, int p, int line_base,
-                                 int lane_in_line, bool live_line, T* fl, T* kl, T* sl) {
+__device__ inline void ens_lines(const EnsStageArgs& ea, const EnsInst& I, int p, int lane_in_line,
+                                 bool live_line, T* fl, T* kl, T* sl) {
   const int n1 = I.n1;
   const int j1 = p % I.n_p1, j23 = p / I.n_p1, j2 = j23 % I.n_p2, j3 = j23 / I.n_p2;
   const T C1 = T(centre(I.p1s, j1, I.dp1));
@@ -74,38 +80,37 @@ __device__ inline void ens_lines(const EnsStageArgs& ea, const EnsInst& I, int p
   for (int k = 0; k < NR; ++k) {
     i1[k] = lane_in_line + 64 * k;
     live[k] = live_line && i1[k] < n1;
+    if (!live[k]) i1[k] = 0;
     f[k] = live[k] ? u[i1[k]] : T(0);
-    if (live[k]) fl[line_base + i1[k]] = f[k];
+    ql_put(fl, i1[k], live[k], n1, f[k]);
   }
   __builtin_amdgcn_wave_barrier();
-  auto at = [&](int i, int o) { return line_base + qm_mod(i + o, n1); };
   T m3[NR], m2[NR], m1[NR], p1[NR], p2[NR], kap[NR];
 #pragma unroll
   for (int k = 0; k < NR; ++k) {
-    const int i = i1[k];
-    m3[k] = fl[at(i, -3)];
-    m2[k] = fl[at(i, -2)];
-    m1[k] = fl[at(i, -1)];
-    p1[k] = fl[at(i, 1)];
-    p2[k] = fl[at(i, 2)];
+    const T* line = fl + i1[k];
+    m3[k] = line[-3];
+    m2[k] = line[-2];
+    m1[k] = line[-1];
+    p1[k] = line[1];
+    p2[k] = line[2];
     kap[k] = ql_kappa<R, T>(m1[k], f[k], p1[k]);
-    if (live[k]) kl[at(i, 0)] = kap[k];
+    ql_put(kl, i1[k], live[k], n1, kap[k]);
   }
   __builtin_amdgcn_wave_barrier();
   T s[NR];
 #pragma unroll
   for (int k = 0; k < NR; ++k) {
-    const int i = i1[k];
     T km2 = T(0), km1 = T(0), kp1 = T(0);
     if constexpr (R == RC_WENO5) {
-      km2 = kl[at(i, -2)];
-      km1 = kl[at(i, -1)];
-      kp1 = kl[at(i, 1)];
+      km2 = kl[i1[k] - 2];
+      km1 = kl[i1[k] - 1];
+      kp1 = kl[i1[k] + 1];
     }
     // lower face of cell i: right face of i-1 (C > 0) or left face of i
     s[k] = pos ? ql_face<R, true, T>(m3[k], m2[k], m1[k], f[k], p1[k], km2, km1, kap[k])
                : ql_face<R, false, T>(m2[k], m1[k], f[k], p1[k], p2[k], km1, kap[k], kp1);
-    if (live[k]) sl[at(i, 0)] = s[k];
+    ql_put(sl, i1[k], live[k], n1, s[k]);
   }
   __builtin_amdgcn_wave_barrier();
   const T* tab = (const T*)ea.tab + I.off_tab;
@@ -119,7 +124,7 @@ __device__ inline void ens_lines(const EnsStageArgs& ea, const EnsInst& I, int p
   for (int k = 0; k < NR; ++k) {
     if (!live[k]) continue;
     const int i = i1[k];
-    const T su = sl[at(i, 1)];
+    const T su = sl[i + 1];
     T d = -(C1 * su - C1 * s[k]) * tinv;
     if (I.source) {
       T fmb = tab[(long)j1 * n1 + i];
@@ -138,9 +143,9 @@ __device__ inline void ens_lines(const EnsStageArgs& ea, const EnsInst& I, int p
 // [wg_begin[g], wg_begin[g+1]) with its WPG waves taking every WPG-th task.
 template <typename T, int R>
 __global__ __launch_bounds__(256) void ens_stage_kernel(EnsStageArgs ea) {
-  __shared__ T fl[4][256];
-  __shared__ T kl[4][256];
-  __shared__ T sl[4][256];
+  __shared__ T fl[4][ENS_LDS];
+  __shared__ T kl[4][ENS_LDS];
+  __shared__ T sl[4][ENS_LDS];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int nwave = blockDim.x >> 6;
   const int tb = ea.wg_begin[blockIdx.x], te = ea.wg_begin[blockIdx.x + 1];
@@ -148,21 +153,25 @@ __global__ __launch_bounds__(256) void ens_stage_kernel(EnsStageArgs ea) {
     const EnsTile tile = ea.tiles[t];
     const EnsInst I = ea.inst[tile.inst];
     if (I.n1 <= 64) {
-      // 64 / n1 lines per wave, one cell per lane
+      // 64 / n1 lines per wave, one cell per lane; line l starts at
+      // l * (n1 + 2 halo) + halo in LDS
       const int lines = 64 / I.n1;
       const int line = lane / I.n1, c = lane % I.n1;
+      const int off = (line < lines ? line : 0) * (I.n1 + 2 * QL_HALO) + QL_HALO;
       for (int q = 0; q < tile.np; q += lines) {
         const int p = tile.p0 + q + line;
         const bool live = line < lines && q + line < tile.np;
-        ens_lines<T, R, 1>(ea, I, live ? p : tile.p0, line * I.n1, c, live, fl[wave], kl[wave],
-                           sl[wave]);
+        ens_lines<T, R, 1>(ea, I, live ? p : tile.p0, c, live, fl[wave] + off, kl[wave] + off,
+                           sl[wave] + off);
       }
     } else if (I.n1 <= 128) {
       for (int q = 0; q < tile.np; ++q)
-        ens_lines<T, R, 2>(ea, I, tile.p0 + q, 0, lane, true, fl[wave], kl[wave], sl[wave]);
+        ens_lines<T, R, 2>(ea, I, tile.p0 + q, lane, true, fl[wave] + QL_HALO,
+                           kl[wave] + QL_HALO, sl[wave] + QL_HALO);
     } else {
       for (int q = 0; q < tile.np; ++q)
-        ens_lines<T, R, 4>(ea, I, tile.p0 + q, 0, lane, true, fl[wave], kl[wave], sl[wave]);
+        ens_lines<T, R, 4>(ea, I, tile.p0 + q, lane, true, fl[wave] + QL_HALO,
+                           kl[wave] + QL_HALO, sl[wave] + QL_HALO);
     }
   }
 }

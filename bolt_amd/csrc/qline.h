@@ -44,7 +44,7 @@ struct QLParams {
 // sum-of-squares indicator (weno5.py:58-80), whose squares would underflow in
 // products.  Other reconstructions keep nothing.
 template <int R, typename T>
-__device__ inline T ql_kappa(T a, T b, T c) {
+__host__ __device__ inline T ql_kappa(T a, T b, T c) {
   if constexpr (R != RC_WENO5) {
     return T(0);
   } else if constexpr (sizeof(T) == 8) {
@@ -52,7 +52,11 @@ __device__ inline T ql_kappa(T a, T b, T c) {
     return bt * bt;
   } else {
     const float bt = weno_beta_sos(a, b, c);
+#if defined(__HIP_DEVICE_COMPILE__)
     return __builtin_amdgcn_rcpf(16.0f * bt * bt);
+#else
+    return 1.0f / (16.0f * bt * bt);
+#endif
   }
 }
 
@@ -63,7 +67,7 @@ __device__ inline T ql_kappa(T a, T b, T c) {
 // units of 1/16; the d factors of the numerator are folded into the
 // substencil coefficients (all exact in binary).
 template <int R, bool RIGHT, typename T>
-__device__ inline T ql_face(T y0, T y1, T y2, T y3, T y4, T ka, T kb, T kc) {
+__host__ __device__ inline T ql_face(T y0, T y1, T y2, T y3, T y4, T ka, T kb, T kc) {
   if constexpr (R == RC_WENO5) {
     T u1, u2, u3;
     if constexpr (RIGHT) {
@@ -93,6 +97,20 @@ __device__ inline T ql_face(T y0, T y1, T y2, T y3, T y4, T ka, T kb, T kc) {
   }
 }
 
+// Halo of the LDS q1 lines: cells -3..-1 and n1..n1+2 hold periodic copies,
+// so every neighbour read is the lane's own address plus a constant.
+constexpr int QL_HALO = 3;
+
+// Store v for cell i of a halo-padded line (n1 >= QL_HALO), with its
+// periodic copies.
+template <typename T>
+__device__ inline void ql_put(T* line, int i, bool live, int n1, T v) {
+  if (!live) return;
+  line[i] = v;
+  if (i < QL_HALO) line[i + n1] = v;
+  if (i >= n1 - QL_HALO) line[i - n1] = v;
+}
+
 // Stage forms the march is specialised for (out = a*un + b*(w + cdt*L(u))):
 // QL_WU_PLAIN  w == u, a == 0, b == 1  (first stage of either stepper)
 // QL_WU_COMB   w == u, a != 0          (SSP-RK3 stages 2 and 3)
@@ -100,6 +118,10 @@ __device__ inline T ql_face(T y0, T y1, T y2, T y3, T y4, T ka, T kb, T kc) {
 // QL_GENERAL   anything else, decided at run time.
 enum QlMode : int { QL_WU_PLAIN = 0, QL_WU_COMB = 1, QL_W_PLAIN = 2, QL_GENERAL = 3 };
 
+// Every f-sized stream (window rows of u, w, un, the output) is read or
+// written once per launch, so it goes through non-temporal accesses and the
+// L2 keeps the Maxwellian table rows the p-cells of a segment share.
+//
 // One wave's march over a q2 segment for the upwind directions P1 (C_q1 > 0)
 // and P2 (C_q2 > 0), fixed at compile time, in stage form MODE with PD
 // Maxwellian factors per cell (0 = no source; -1 = decided at run time).  Along q2 the register window
@@ -138,19 +160,15 @@ __device__ __forceinline__ void ql_march(const QLParams& q, int p, int seg, int 
   const bool src = pd >= 1, src2 = pd >= 2, src3 = pd >= 3;
   constexpr int B = P2 ? -2 : -1;  // window row offset from the current row
 
-  // lane cells and the LDS indices of their periodic q1 neighbours
-  int i1[NR], na[NR], nb[NR], nc[NR], nd[NR];
+  // lane cells; their periodic q1 neighbours are read from the halo-padded
+  // LDS lines at constant offsets
+  int i1[NR];
   bool live[NR];
 #pragma unroll
   for (int k = 0; k < NR; ++k) {
     i1[k] = lane + 64 * k;
     live[k] = i1[k] < n1;
     if (!live[k]) i1[k] = 0;
-    // P1: i-3, i-2, i-1, i+1; otherwise i-2, i-1, i+1, i+2
-    na[k] = qm_mod(i1[k] + (P1 ? -3 : -2), n1);
-    nb[k] = qm_mod(i1[k] + (P1 ? -2 : -1), n1);
-    nc[k] = qm_mod(i1[k] + (P1 ? -1 : 1), n1);
-    nd[k] = qm_mod(i1[k] + (P1 ? 1 : 2), n1);
   }
 
   T y[NR][5], ka[NR], kb[NR], s2m[NR], wc[NR], unc[NR], tc1[NR], tc2[NR], tc3[NR];
@@ -194,6 +212,8 @@ __device__ __forceinline__ void ql_march(const QLParams& q, int p, int seg, int 
   int rnxw = qm_mod(r0 + B + 5, n2);
 
   for (int it = 0; it < q.L; ++it) {
+    // the next row's pointwise inputs and window row, one row ahead, so
+    // loads stay in flight across the stencil work
     T yn[NR], wn[NR], unn[NR], tn1[NR], tn2[NR], tn3[NR];
     {
       const T* uw = u + rnxw * rowP;
@@ -202,9 +222,9 @@ __device__ __forceinline__ void ql_march(const QLParams& q, int p, int seg, int 
       const long tq = (long)rnx1 * n1;
 #pragma unroll
       for (int k = 0; k < NR; ++k) {
-        yn[k] = uw[i1[k]];
-        wn[k] = w_is_u ? T(0) : w1[i1[k]];
-        unn[k] = use_un ? un1[i1[k]] : T(0);
+        yn[k] = __builtin_nontemporal_load(uw + i1[k]);
+        wn[k] = w_is_u ? T(0) : __builtin_nontemporal_load(w1 + i1[k]);
+        unn[k] = use_un ? __builtin_nontemporal_load(un1 + i1[k]) : T(0);
         tn1[k] = src ? t1p[tq + i1[k]] : T(0);
         tn2[k] = src2 ? t2p[tq + i1[k]] : T(0);
         tn3[k] = src3 ? t3p[tq + i1[k]] : T(0);
@@ -229,19 +249,20 @@ __device__ __forceinline__ void ql_march(const QLParams& q, int p, int seg, int 
     }
     if (do_q1) {
 #pragma unroll
-      for (int k = 0; k < NR; ++k)
-        if (live[k]) fl[i1[k]] = fc[k];
+      for (int k = 0; k < NR; ++k) ql_put(fl, i1[k], live[k], n1, fc[k]);
       __builtin_amdgcn_wave_barrier();
+      // P1: cells i-3, i-2, i-1, i+1; otherwise i-2, i-1, i+1, i+2
       T va[NR], vb[NR], vc[NR], vd[NR], kown[NR];
 #pragma unroll
       for (int k = 0; k < NR; ++k) {
-        va[k] = fl[na[k]];
-        vb[k] = fl[nb[k]];
-        vc[k] = fl[nc[k]];
-        vd[k] = fl[nd[k]];
+        const T* line = fl + i1[k];
+        va[k] = line[P1 ? -3 : -2];
+        vb[k] = line[P1 ? -2 : -1];
+        vc[k] = line[P1 ? -1 : 1];
+        vd[k] = line[P1 ? 1 : 2];
         // neighbours i-1 and i+1 of the cell: (vc, vd) for P1, (vb, vc) otherwise
         kown[k] = P1 ? ql_kappa<R, T>(vc[k], fc[k], vd[k]) : ql_kappa<R, T>(vb[k], fc[k], vc[k]);
-        if (live[k]) kl[i1[k]] = kown[k];
+        ql_put(kl, i1[k], live[k], n1, kown[k]);
       }
       __builtin_amdgcn_wave_barrier();
       T s[NR];
@@ -251,25 +272,25 @@ __device__ __forceinline__ void ql_march(const QLParams& q, int p, int seg, int 
           // right face of cell i-1: window i-3..i+1, indicators of i-2, i-1, i
           T k2 = T(0), k1 = T(0);
           if constexpr (R == RC_WENO5) {
-            k2 = kl[nb[k]];
-            k1 = kl[nc[k]];
+            k2 = kl[i1[k] - 2];
+            k1 = kl[i1[k] - 1];
           }
           s[k] = ql_face<R, true, T>(va[k], vb[k], vc[k], fc[k], vd[k], k2, k1, kown[k]);
         } else {
           // left face of cell i: window i-2..i+2, indicators of i-1, i, i+1
           T k1 = T(0), kp = T(0);
           if constexpr (R == RC_WENO5) {
-            k1 = kl[nb[k]];
-            kp = kl[nc[k]];
+            k1 = kl[i1[k] - 1];
+            kp = kl[i1[k] + 1];
           }
           s[k] = ql_face<R, false, T>(va[k], vb[k], fc[k], vc[k], vd[k], k1, kown[k], kp);
         }
-        if (live[k]) sl[i1[k]] = s[k];
+        ql_put(sl, i1[k], live[k], n1, s[k]);
       }
       __builtin_amdgcn_wave_barrier();
 #pragma unroll
       for (int k = 0; k < NR; ++k) {
-        const T su = sl[P1 ? nd[k] : nc[k]];  // lower face of cell i+1
+        const T su = sl[i1[k] + 1];  // lower face of cell i+1
         d[k] = -(C1 * su - C1 * s[k]) * tinv1 - d[k];
       }
       __builtin_amdgcn_wave_barrier();
@@ -291,7 +312,7 @@ __device__ __forceinline__ void ql_march(const QLParams& q, int p, int seg, int 
       }
       const T tv = (w_is_u ? f : wc[k]) + dk * tcdt;
       const T x = use_un ? ta * unc[k] + tb * tv : (plain ? tv : tb * tv);
-      if (live[k]) orow[i1[k]] = x;
+      if (live[k]) __builtin_nontemporal_store(x, orow + i1[k]);
 #pragma unroll
       for (int j = 0; j < 4; ++j) y[k][j] = y[k][j + 1];
       y[k][4] = yn[k];
@@ -317,9 +338,11 @@ __device__ __forceinline__ void ql_march(const QLParams& q, int p, int seg, int 
 // Maxwellian tables stay in that XCD's L2.
 template <typename T, int R, int NR, int MODE, int PD>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void qline_kernel(QLParams q) {
-  __shared__ T fl[4][NR * 64];  // wave-private q1 lines: row values,
-  __shared__ T kl[4][NR * 64];  // q1 indicator quantities,
-  __shared__ T sl[4][NR * 64];  // q1 lower-face states
+  // wave-private q1 lines with three halo cells on each side: row values,
+  // q1 indicator quantities and lower-face states
+  __shared__ T fl[4][NR * 64 + 2 * QL_HALO];
+  __shared__ T kl[4][NR * 64 + 2 * QL_HALO];
+  __shared__ T sl[4][NR * 64 + 2 * QL_HALO];
   // the wave index is uniform across the wave: keep it (and all that derives
   // from it: p-cell, velocities, row pointers) in scalar registers
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -332,9 +355,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void q
   const int j1 = p % q.n_p1, j2 = (p / q.n_p1) % q.n_p2;
   const bool pos1 = centre(q.p1s, j1, q.dp1) > 0.0;  // wave-uniform
   const bool pos2 = centre(q.p2s, j2, q.dp2) > 0.0;
-  T* f = fl[wave];
-  T* k = kl[wave];
-  T* s = sl[wave];
+  T* f = fl[wave] + QL_HALO;
+  T* k = kl[wave] + QL_HALO;
+  T* s = sl[wave] + QL_HALO;
   if (pos1) {
     if (pos2) ql_march<T, R, NR, true, true, MODE, PD>(q, p, seg, lane, f, k, s);
     else ql_march<T, R, NR, true, false, MODE, PD>(q, p, seg, lane, f, k, s);

@@ -109,7 +109,7 @@ __global__ __launch_bounds__(256) void mline_vec_kernel(const T* f, int n1, int 
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int pu = p + u * D;
-      x[u] = pu < pe ? *(const V*)(base + (long)pu * n1) : V(0);
+      x[u] = pu < pe ? __builtin_nontemporal_load((const V*)(base + (long)pu * n1)) : V(0);
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {

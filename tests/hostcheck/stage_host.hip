@@ -156,3 +156,42 @@ extern "C" int host_adjoint_stage(const boltx_grid* g, const boltx_physics* ph,
   }
   return 0;
 }
+
+#include "../../bolt_amd/csrc/qline.h"
+
+// Faces of the q1-line kernels (indicator quantities per cell, one division
+// per face) on n five-cell windows y[5*i .. 5*i+4]: the right (or left) face
+// of the centre cell.
+template <typename T, int R>
+static void ql_faces(const double* y, double* out, int n, bool right) {
+  for (int i = 0; i < n; ++i) {
+    T a[5];
+    for (int k = 0; k < 5; ++k) a[k] = T(y[5 * i + k]);
+    const T ka = ql_kappa<R, T>(a[0], a[1], a[2]);
+    const T kb = ql_kappa<R, T>(a[1], a[2], a[3]);
+    const T kc = ql_kappa<R, T>(a[2], a[3], a[4]);
+    out[i] = double(right ? ql_face<R, true, T>(a[0], a[1], a[2], a[3], a[4], ka, kb, kc)
+                          : ql_face<R, false, T>(a[0], a[1], a[2], a[3], a[4], ka, kb, kc));
+  }
+}
+
+extern "C" int host_ql_faces(int recon, int precision, int right, const double* y, double* out,
+                             int n) {
+  const bool r = right != 0;
+  if (precision == 0) {
+    switch (recon) {
+      case 0: ql_faces<double, RC_PC>(y, out, n, r); break;
+      case 1: ql_faces<double, RC_MINMOD>(y, out, n, r); break;
+      case 2: ql_faces<double, RC_PPM>(y, out, n, r); break;
+      default: ql_faces<double, RC_WENO5>(y, out, n, r);
+    }
+  } else {
+    switch (recon) {
+      case 0: ql_faces<float, RC_PC>(y, out, n, r); break;
+      case 1: ql_faces<float, RC_MINMOD>(y, out, n, r); break;
+      case 2: ql_faces<float, RC_PPM>(y, out, n, r); break;
+      default: ql_faces<float, RC_WENO5>(y, out, n, r);
+    }
+  }
+  return 0;
+}

@@ -4,6 +4,9 @@ same state.  Tolerances: FP64 <= 1e-12 relative L-inf of f (the per-step
 parity bar); FP32-bulk <= 1e-5 relative L2 against the FP64 oracle."""
 import numpy as np
 import pytest
+import torch
+
+from oracle import bolt_oracle as bo
 
 from tests.cases import case_2d2v, case_c1, case_2d3v, oracle_solver
 from tests.hostcheck import harness as H
@@ -144,3 +147,34 @@ def test_slab_kernel_flux_matches_reference_form(rq, rs, precision):
     scale = np.abs(C) * np.abs(a7).max(axis=1)
     tol = 1e-14 if precision == 0 else 2e-6
     assert np.all(np.abs(fast - ref) <= tol * scale)
+
+
+@pytest.mark.parametrize("rq", ["minmod", "ppm", "weno5"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_qline_faces_match_reference_faces(rq, precision):
+    """The q1-line kernels' face arithmetic (indicator quantity per cell, the
+    d-weights folded into the substencils, one division per face) against the
+    oracle's faces (weno5.py:57-98, ppm.py:57-98, minmod.py:64-71) on random
+    windows: 1e-13 relative in FP64, 1e-5 in FP32."""
+    import ctypes
+    from tests.hostcheck import harness as H
+    lib = H.lib()
+    lib.host_ql_faces.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.c_int]
+    rng = np.random.default_rng(11)
+    n = 4000
+    y = np.ascontiguousarray(1.0 + 0.3 * rng.standard_normal((n, 5)))
+    T5 = [torch.from_numpy(y[:, k].copy()) for k in range(5)]
+    if rq == "weno5":
+        L, R = bo.weno5_faces(*T5)
+    elif rq == "ppm":
+        L, R = bo.ppm_faces(*T5)
+    else:
+        x = torch.from_numpy(y[:, 2].copy())
+        s = bo.minmod(2 * (x - T5[1]), 0.5 * (T5[3] - T5[1]), 2 * (T5[3] - x))
+        L, R = x - 0.5 * s, x + 0.5 * s
+    for right, ref in ((1, R.numpy()), (0, L.numpy())):
+        out = np.empty(n)
+        lib.host_ql_faces(H.L.RECON[rq], precision, right, y.ctypes.data, out.ctypes.data, n)
+        err = np.abs(out - ref).max() / np.abs(ref).max()
+        assert err <= (1e-13 if precision == 0 else 1e-5), (rq, right, err)

@@ -55,3 +55,31 @@ def test_invalid_arguments_are_rejected():
     rc = lib.boltx_moments(ctypes.byref(g2), ctypes.byref(ph), ctypes.c_void_p(1),
                            ctypes.c_void_p(1), ctypes.c_void_p(1), None)
     assert rc == -3 and b"256" in lib.boltx_last_error()
+
+
+def test_adjoint_and_ensemble_reject_bad_input():
+    """Argument checks of the adjoint and ensemble entry points, before any
+    device work."""
+    import ctypes as C
+    lib = _lib.load()
+    g = _lib.Grid(8, 8, 6, 6, 1, 1, 0, 0.125, 0.125, 1.0, 1.0, 1.0, -3.0, -3.0, -0.5)
+    ph = _lib.Physics(3, 3, 1, 1, 1, 0, 2, 1, 1.0, -1.0, 1.0, 0.01)  # FP32-bulk
+    adj = _lib.Adjoint(C.c_void_p(16), None, C.c_void_p(32), None, C.c_void_p(48),
+                       0.0, 1.0, 1.0, None, None)
+    assert lib.boltx_adjoint_stage(C.byref(g), C.byref(ph), C.byref(adj), None) == -3
+    assert b"FP64" in lib.boltx_last_error()
+    ph.precision = 0
+    assert lib.boltx_adjoint_stage(C.byref(g), C.byref(ph), C.byref(adj), None) == -1
+    assert b"mom_u" in lib.boltx_last_error()
+    adj.out = adj.lam
+    assert lib.boltx_adjoint_stage(C.byref(g), C.byref(ph), C.byref(adj), None) == -1
+    assert b"alias" in lib.boltx_last_error()
+    assert lib.boltx_adjoint_workspace_size(C.byref(g), C.byref(ph)) == 5 * 64 * 8
+    m = (_lib.EnsMember * 1)()
+    m[0].grid = g
+    m[0].grid.n_q2 = 1
+    m[0].phys = ph
+    m[0].dt = 0.0
+    t = _lib.EnsTuning(4, 1024, 4, 16384)
+    assert lib.boltx_ensemble_step(1, m, C.byref(t), 1, None, None, None, None, 1, None) == -1
+    assert b"dt" in lib.boltx_last_error()
