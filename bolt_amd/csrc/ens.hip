@@ -176,7 +176,7 @@ int moments_and_tables(const Plan& pl, char* ws, const void* f, hipStream_t s) {
                      (double*)(ws + pl.o_part));
   int rc = launch_status("ens_moments_kernel");
   if (rc) return rc;
-  hipLaunchKernelGGL(ens_finalize_kernel<T>, dim3((unsigned)pl.inst.size()), dim3(256), 0, s,
+  hipLaunchKernelGGL(ens_finalize_kernel<T>, dim3((unsigned)pl.inst.size(), ENS_TGROUPS), dim3(256), 0, s,
                      inst, (const double*)(ws + pl.o_part), (double*)(ws + pl.o_mom),
                      (T*)(ws + pl.o_tab), (int*)(ws + pl.o_flag));
   return launch_status("ens_finalize_kernel");

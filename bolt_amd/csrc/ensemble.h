@@ -64,26 +64,56 @@ constexpr int ENS_LDS = 264;
 // works on cells i1 = c0 + 64 k (k < NR) of the line starting at ``line``
 // (LDS, halo-padded; p-cell p), all lanes of a line sharing C1 and the table
 // rows.
+// The values of u on a lane's cells of the line of p-cell p (0 where dead).
+template <typename T, int NR>
+__device__ inline void ens_load_line(const EnsStageArgs& ea, const EnsInst& I, int p,
+                                     int lane_in_line, bool live_line, T f[NR]) {
+  const T* u = (const T*)ea.u + I.off_f + (long)p * I.n1;
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    const int i = lane_in_line + 64 * k;
+    f[k] = live_line && i < I.n1 ? __builtin_nontemporal_load(u + i) : T(0);
+  }
+}
+
 template <typename T, int R, int NR>
-__device__ inline void ens_lines(const EnsStageArgs& ea, const EnsInst& I, int p, int lane_in_line,
-                                 bool live_line, T* fl, T* kl, T* sl) {
+__device__ inline void ens_lines(const EnsStageArgs& ea, const EnsInst& I, int p, int j1, int j2,
+                                 int j3, int lane_in_line, bool live_line, const T f[NR], T* fl,
+                                 T* kl, T* sl) {
   const int n1 = I.n1;
-  const int j1 = p % I.n_p1, j23 = p / I.n_p1, j2 = j23 % I.n_p2, j3 = j23 / I.n_p2;
   const T C1 = T(centre(I.p1s, j1, I.dp1));
   const bool pos = C1 > T(0);
   const long base = I.off_f + (long)p * n1;
-  const T* u = (const T*)ea.u + base;
   int i1[NR];
   bool live[NR];
-  T f[NR];
 #pragma unroll
   for (int k = 0; k < NR; ++k) {
     i1[k] = lane_in_line + 64 * k;
     live[k] = live_line && i1[k] < n1;
     if (!live[k]) i1[k] = 0;
-    f[k] = live[k] ? u[i1[k]] : T(0);
-    ql_put(fl, i1[k], live[k], n1, f[k]);
   }
+  // pointwise inputs, issued before the LDS work so it covers their latency
+  const T* tab = (const T*)ea.tab + I.off_tab;
+  const T* wv = (const T*)ea.w + base;
+  const T* unv = (const T*)ea.un + base;
+  const bool w_is_u = ea.w == ea.u;
+  T fmb[NR], wc[NR], unc[NR];
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    const int i = i1[k];
+    fmb[k] = wc[k] = unc[k] = T(0);
+    if (live[k]) {
+      if (I.source) {
+        fmb[k] = tab[(long)j1 * n1 + i];
+        if (I.p_dim >= 2) fmb[k] = fmb[k] * tab[(long)(I.n_p1 + j2) * n1 + i];
+        if (I.p_dim >= 3) fmb[k] = fmb[k] * tab[(long)(I.n_p1 + I.n_p2 + j3) * n1 + i];
+      }
+      if (!w_is_u) wc[k] = __builtin_nontemporal_load(wv + i);
+      if (ea.use_un) unc[k] = __builtin_nontemporal_load(unv + i);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NR; ++k) ql_put(fl, i1[k], live[k], n1, f[k]);
   __builtin_amdgcn_wave_barrier();
   T m3[NR], m2[NR], m1[NR], p1[NR], p2[NR], kap[NR];
 #pragma unroll
@@ -113,29 +143,46 @@ __device__ inline void ens_lines(const EnsStageArgs& ea, const EnsInst& I, int p
     ql_put(sl, i1[k], live[k], n1, s[k]);
   }
   __builtin_amdgcn_wave_barrier();
-  const T* tab = (const T*)ea.tab + I.off_tab;
   const T tinv = T(I.inv_dq), ttau = T(I.inv_tau), tcdt = T(I.cdt * ea.cscale);
   const T ta = T(ea.a), tb = T(ea.b);
   T* out = (T*)ea.out + base;
-  const T* wv = (const T*)ea.w + base;
-  const T* unv = (const T*)ea.un + base;
-  const bool w_is_u = ea.w == ea.u;
 #pragma unroll
   for (int k = 0; k < NR; ++k) {
     if (!live[k]) continue;
     const int i = i1[k];
     const T su = sl[i + 1];
     T d = -(C1 * su - C1 * s[k]) * tinv;
-    if (I.source) {
-      T fmb = tab[(long)j1 * n1 + i];
-      if (I.p_dim >= 2) fmb = fmb * tab[(long)(I.n_p1 + j2) * n1 + i];
-      if (I.p_dim >= 3) fmb = fmb * tab[(long)(I.n_p1 + I.n_p2 + j3) * n1 + i];
-      d = d + (fmb - f[k]) * ttau;
-    }
-    const T tv = (w_is_u ? f[k] : wv[i]) + d * tcdt;
-    out[i] = ea.use_un ? ta * unv[i] + tb * tv : (ea.b_one ? tv : tb * tv);
+    if (I.source) d = d + (fmb[k] - f[k]) * ttau;
+    const T tv = (w_is_u ? f[k] : wc[k]) + d * tcdt;
+    __builtin_nontemporal_store(ea.use_un ? ta * unc[k] + tb * tv : (ea.b_one ? tv : tb * tv),
+                                out + i);
   }
   __builtin_amdgcn_wave_barrier();
+}
+
+// A wave task of lines longer than 64 cells: one line at a time, the next
+// line's values loaded while the current one is worked on.
+template <typename T, int R, int NR>
+__device__ inline void ens_task(const EnsStageArgs& ea, const EnsInst& I, const EnsTile& tile,
+                                int lane, T* fl, T* kl, T* sl) {
+  int j1 = tile.p0 % I.n_p1, j23 = tile.p0 / I.n_p1;
+  int j2 = j23 % I.n_p2, j3 = j23 / I.n_p2;
+  T cur[NR], nxt[NR];
+  ens_load_line<T, NR>(ea, I, tile.p0, lane, true, cur);
+  for (int q = 0; q < tile.np; ++q) {
+    const bool more = q + 1 < tile.np;
+    ens_load_line<T, NR>(ea, I, more ? tile.p0 + q + 1 : tile.p0, lane, more, nxt);
+    ens_lines<T, R, NR>(ea, I, tile.p0 + q, j1, j2, j3, lane, true, cur, fl, kl, sl);
+#pragma unroll
+    for (int k = 0; k < NR; ++k) cur[k] = nxt[k];
+    if (++j1 == I.n_p1) {
+      j1 = 0;
+      if (++j2 == I.n_p2) {
+        j2 = 0;
+        ++j3;
+      }
+    }
+  }
 }
 
 #ifndef BX_NO_KERNELS
@@ -158,69 +205,129 @@ __global__ __launch_bounds__(256) void ens_stage_kernel(EnsStageArgs ea) {
       const int lines = 64 / I.n1;
       const int line = lane / I.n1, c = lane % I.n1;
       const int off = (line < lines ? line : 0) * (I.n1 + 2 * QL_HALO) + QL_HALO;
+      // p-cell of this lane's line and its (j1, j2, j3), carried without
+      // divisions as the task advances by ``lines`` p-cells
+      const int p_first = tile.p0 + (line < lines ? line : 0);
+      int j1 = p_first % I.n_p1, j23 = p_first / I.n_p1;
+      int j2 = j23 % I.n_p2, j3 = j23 / I.n_p2;
+      T cur[1], nxt[1];
+      ens_load_line<T, 1>(ea, I, p_first, c, line < lines && line < tile.np, cur);
       for (int q = 0; q < tile.np; q += lines) {
-        const int p = tile.p0 + q + line;
         const bool live = line < lines && q + line < tile.np;
-        ens_lines<T, R, 1>(ea, I, live ? p : tile.p0, c, live, fl[wave] + off, kl[wave] + off,
-                           sl[wave] + off);
+        const bool more = line < lines && q + lines + line < tile.np;
+        ens_load_line<T, 1>(ea, I, more ? tile.p0 + q + lines + line : tile.p0, c, more, nxt);
+        ens_lines<T, R, 1>(ea, I, live ? tile.p0 + q + line : tile.p0, j1, j2, j3, c, live, cur,
+                           fl[wave] + off, kl[wave] + off, sl[wave] + off);
+        cur[0] = nxt[0];
+        j1 += lines;
+        while (j1 >= I.n_p1) {
+          j1 -= I.n_p1;
+          if (++j2 == I.n_p2) {
+            j2 = 0;
+            ++j3;
+          }
+        }
       }
     } else if (I.n1 <= 128) {
-      for (int q = 0; q < tile.np; ++q)
-        ens_lines<T, R, 2>(ea, I, tile.p0 + q, lane, true, fl[wave] + QL_HALO,
-                           kl[wave] + QL_HALO, sl[wave] + QL_HALO);
+      ens_task<T, R, 2>(ea, I, tile, lane, fl[wave] + QL_HALO, kl[wave] + QL_HALO,
+                        sl[wave] + QL_HALO);
     } else {
-      for (int q = 0; q < tile.np; ++q)
-        ens_lines<T, R, 4>(ea, I, tile.p0 + q, lane, true, fl[wave] + QL_HALO,
-                           kl[wave] + QL_HALO, sl[wave] + QL_HALO);
+      ens_task<T, R, 4>(ea, I, tile, lane, fl[wave] + QL_HALO, kl[wave] + QL_HALO,
+                        sl[wave] + QL_HALO);
     }
   }
 }
 
-// Partial moments of one (instance, p-chunk): thread t sums q1 cell t % n1
-// over every (256/n1)-th p-cell of the chunk; the p-groups are added in a
-// fixed order.
+// Partial moments of one (instance, p-chunk): the chunk is a contiguous
+// span of (p1 - p0) * n1 elements, streamed in 16-byte vectors (non-temporal,
+// four in flight) when n1 is a multiple of the vector width, each thread
+// keeping the same VEC q1 cells; otherwise one element per thread.  The
+// partial sums of threads holding the same cells are added in a fixed order.
 template <typename T>
 __global__ __launch_bounds__(256) void ens_moments_kernel(const EnsInst* inst,
                                                           const EnsChunk* chunks, const T* f,
                                                           double* part) {
+  constexpr int VEC = 16 / sizeof(T);
+  typedef T V __attribute__((ext_vector_type(VEC)));
   __shared__ double red[5][256];
   const EnsChunk ch = chunks[blockIdx.x];
   const EnsInst I = inst[ch.inst];
   const int t = threadIdx.x, n1 = I.n1;
-  const int groups = 256 / n1;
-  const int q = t % n1, g = t / n1;
-  double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
+  const bool vec = n1 % VEC == 0 && I.off_f % VEC == 0;  // 16-byte aligned rows
+  const int w = vec ? VEC : 1;          // cells per thread
+  const int lpl = n1 / w;               // threads per line
+  const int groups = 256 / lpl;         // p-cells per workgroup step
+  const int g = t / lpl, c0 = (t % lpl) * w;
+  double a[5][VEC];
+#pragma unroll
+  for (int k = 0; k < 5; ++k)
+#pragma unroll
+    for (int c = 0; c < VEC; ++c) a[k][c] = 0.0;
   if (g < groups) {
-    const T* col = f + I.off_f + q;
-    for (int p = ch.p0 + g; p < ch.p1; p += groups) {
-      const int j1 = p % I.n_p1, j23 = p / I.n_p1, j2 = j23 % I.n_p2, j3 = j23 / I.n_p2;
+    const T* col = f + I.off_f + c0;
+    const int pf = ch.p0 + g;
+    int j1 = pf % I.n_p1, j23 = pf / I.n_p1;
+    int j2 = j23 % I.n_p2, j3 = j23 / I.n_p2;
+    for (int p = pf; p < ch.p1; p += groups) {
       const double v1 = centre(I.p1s, j1, I.dp1), v2 = centre(I.p2s, j2, I.dp2);
       const double v3 = centre(I.p3s, j3, I.dp3);
-      const double x = double(col[(long)p * n1]);
-      a0 += x;
-      a1 += x * v1;
-      a2 += x * v2;
-      a3 += x * v3;
-      a4 += x * (0.5 * ((v1 * v1 + v2 * v2) + v3 * v3));
+      const double e = 0.5 * ((v1 * v1 + v2 * v2) + v3 * v3);
+      double x[VEC];
+      if (vec) {
+        const V xv = __builtin_nontemporal_load((const V*)(col + (long)p * n1));
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) x[c] = double(xv[c]);
+      } else {
+        x[0] = double(col[(long)p * n1]);
+#pragma unroll
+        for (int c = 1; c < VEC; ++c) x[c] = 0.0;
+      }
+#pragma unroll
+      for (int c = 0; c < VEC; ++c) {
+        a[0][c] += x[c];
+        a[1][c] += x[c] * v1;
+        a[2][c] += x[c] * v2;
+        a[3][c] += x[c] * v3;
+        a[4][c] += x[c] * e;
+      }
+      j1 += groups;
+      while (j1 >= I.n_p1) {
+        j1 -= I.n_p1;
+        if (++j2 == I.n_p2) {
+          j2 = 0;
+          ++j3;
+        }
+      }
     }
   }
-  red[0][t] = a0;
-  red[1][t] = a1;
-  red[2][t] = a2;
-  red[3][t] = a3;
-  red[4][t] = a4;
-  __syncthreads();
-  if (t < n1) {
-    for (int k = 0; k < 5; ++k) {
-      double s = 0.0;
-      for (int gg = 0; gg < groups; ++gg) s += red[k][gg * n1 + t];
-      part[I.off_part + ((long)ch.chunk * 5 + k) * n1 + t] = s;
+  __shared__ double cell[5][256];
+  // per thread the partial of each of its cells, then a fixed-order sum over
+  // the p-groups for every cell of the line
+  for (int c = 0; c < w; ++c) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) red[k][t] = a[k][c];
+    __syncthreads();
+    if (t < lpl) {
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        double s = 0.0;
+        for (int gg = 0; gg < groups; ++gg) s += red[k][gg * lpl + t];
+        cell[k][t * w + c] = s;
+      }
     }
+    __syncthreads();
   }
+  if (t < n1)
+    for (int k = 0; k < 5; ++k) part[I.off_part + ((long)ch.chunk * 5 + k) * n1 + t] = cell[k][t];
 }
 
-// Moment block and Maxwellian tables of every instance (one workgroup per
-// instance, one thread per q1 cell); a non-finite density sets *flag.
+// Moment block and Maxwellian tables of every instance: workgroup (instance
+// blockIdx.x, table group blockIdx.y of ENS_TGROUPS), one thread per q1 cell.
+// Every group sums the partials of its q-cells (fixed order); group 0 writes
+// the moment block and the divergence flag; group g writes the table rows
+// j = g, g + ENS_TGROUPS, ...
+constexpr int ENS_TGROUPS = 8;
+
 template <typename T>
 __global__ __launch_bounds__(256) void ens_finalize_kernel(const EnsInst* inst,
                                                            const double* part, double* mom,
@@ -230,26 +337,26 @@ __global__ __launch_bounds__(256) void ens_finalize_kernel(const EnsInst* inst,
   if (q >= I.n1) return;
   const int n1 = I.n1;
   double m[5];
-  for (int k = 0; k < 5; ++k) {
-    double s = 0.0;
-    for (int c = 0; c < I.nchunk; ++c) s += part[I.off_part + ((long)c * 5 + k) * n1 + q];
-    m[k] = s * I.measure;
-  }
-  if (!isfinite(m[0])) *flag = 1;
+  sum_moment_partials(part + I.off_part + q, I.nchunk, n1, m);
+  for (int k = 0; k < 5; ++k) m[k] *= I.measure;
   double* mb = mom + I.off_mom;
-  write_moment_block(mb, n1, q, m, I.mass, I.kB, I.p_dim);
+  if (blockIdx.y == 0) {
+    if (!isfinite(m[0])) *flag = 1;
+    write_moment_block(mb, n1, q, m, I.mass, I.kB, I.p_dim);
+  }
   if (!I.source) return;
-  const double* b = mb + MOM_BGK * n1 + q;
-  const double Tq = b[4 * n1];
+  const BgkCell c = bgk_cell(m[0], m[1], m[2], m[3], m[4], I.mass, I.p_dim);
+  const double pref = maxwell_prefactor(c, I.mass, I.kB, I.p_dim);
   T* tb = tab + I.off_tab;
-  for (int j = 0; j < I.n_p1; ++j)
-    tb[(long)j * n1 + q] = T(b[0] * maxwell_factor(centre(I.p1s, j, I.dp1), b[n1], Tq, I.mass, I.kB));
-  for (int j = 0; j < I.n_p2; ++j)
-    tb[(long)(I.n_p1 + j) * n1 + q] =
-        T(maxwell_factor(centre(I.p2s, j, I.dp2), b[2 * n1], Tq, I.mass, I.kB));
-  for (int j = 0; j < I.n_p3; ++j)
-    tb[(long)(I.n_p1 + I.n_p2 + j) * n1 + q] =
-        T(maxwell_factor(centre(I.p3s, j, I.dp3), b[3 * n1], Tq, I.mass, I.kB));
+  const int nj = I.n_p1 + I.n_p2 + I.n_p3;
+  for (int j = blockIdx.y; j < nj; j += ENS_TGROUPS) {
+    double val;
+    if (j < I.n_p1) val = pref * maxwell_factor(centre(I.p1s, j, I.dp1), c.u1, c.T, I.mass, I.kB);
+    else if (j < I.n_p1 + I.n_p2)
+      val = maxwell_factor(centre(I.p2s, j - I.n_p1, I.dp2), c.u2, c.T, I.mass, I.kB);
+    else val = maxwell_factor(centre(I.p3s, j - I.n_p1 - I.n_p2, I.dp3), c.u3, c.T, I.mass, I.kB);
+    tb[(long)j * n1 + q] = T(val);
+  }
 }
 #endif
 

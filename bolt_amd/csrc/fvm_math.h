@@ -264,6 +264,31 @@ __host__ __device__ inline void write_moment_block(double* mom, int Q, int qc, c
   b[6 * Q] = mass / (2.0 * kB * c.T);
 }
 
+// Sum of the five moment partials of one q-cell over nchunk chunks,
+// part[(c * 5 + k) * stride], in chunk order for each k (the same result as
+// the plain loop); loads are issued four chunks at a time.
+__host__ __device__ inline void sum_moment_partials(const double* part, int nchunk, long stride,
+                                                    double m[5]) {
+  double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  int c = 0;
+  for (; c + 4 <= nchunk; c += 4) {
+    double v[4][5];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int k = 0; k < 5; ++k) v[j][k] = part[((long)(c + j) * 5 + k) * stride];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int k = 0; k < 5; ++k) s[k] += v[j][k];
+  }
+  for (; c < nchunk; ++c)
+#pragma unroll
+    for (int k = 0; k < 5; ++k) s[k] += part[((long)c * 5 + k) * stride];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) m[k] = s[k];
+}
+
 // Periodic index: i mod n in [0, n).
 __host__ __device__ inline int qm_mod(int i, int n) {
   i %= n;

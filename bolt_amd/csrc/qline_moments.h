@@ -170,11 +170,8 @@ __global__ void ml_finalize_kernel(const double* part, int nchunk, int Q, double
   const int qc = blockIdx.x * blockDim.x + threadIdx.x;
   if (qc >= Q) return;
   double m[5];
-  for (int k = 0; k < 5; ++k) {
-    double s = 0.0;
-    for (int c = 0; c < nchunk; ++c) s += part[((long)c * 5 + k) * Q + qc];
-    m[k] = s * measure;
-  }
+  sum_moment_partials(part + qc, nchunk, Q, m);
+  for (int k = 0; k < 5; ++k) m[k] *= measure;
   write_moment_block(mom, Q, qc, m, mass, kB, p_dim);
 }
 
