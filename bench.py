@@ -146,7 +146,8 @@ def run_ensemble(steps=200, sample_steps=10):
     """The ensemble workload: 96 1D2V members of uneven sizes (N_q1 in
     {32..256}, N_p in {24..64}^2, tau in [1e-3, 1], own dt each), WENO5 + LLF
     + BGK, SSP-RK3, FP32-bulk with FP64 moments, 200 steps in persistent
-    ensemble launches with the autotuned work-list shape; beside it the same
+    ensemble launches with the autotuned work-list shape, replayed as one HIP
+    graph (the eager time is reported too); beside it the same
     members stepped one after another by the single-problem solver in plain
     FP64 (timed over ``sample_steps`` steps each)."""
     import torch
@@ -164,6 +165,14 @@ def run_ensemble(steps=200, sample_steps=10):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ens.step(steps)
+    torch.cuda.synchronize()
+    t_eager = time.perf_counter() - t0
+    # the same steps as one HIP graph (the launches recorded once, replayed)
+    ens.capture(steps)
+    ens.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.replay()
     torch.cuda.synchronize()
     t_ens = time.perf_counter() - t0
     ens.check_divergence()
@@ -188,6 +197,7 @@ def run_ensemble(steps=200, sample_steps=10):
                         "tau), WENO5 + LLF + BGK, SSP-RK3, FP32-bulk, %d steps" % steps,
             "members": len(dts), "cells": cells, "value": rate,
             "unit": "phase-space cell-updates/s", "ms_per_step": t_ens / steps * 1e3,
+            "ms_per_step_without_graph": t_eager / steps * 1e3,
             "tuning": {"waves_per_group": ens.tuning[0], "cells_per_task": ens.tuning[1],
                        "groups_per_cu": ens.tuning[2], "chunk_cells": ens.tuning[3]},
             "autotune_s": t_tune,

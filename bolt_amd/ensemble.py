@@ -118,6 +118,26 @@ class Ensemble:
             _lib.stream_handle()))
         self.steps_taken += n_steps
 
+    def capture(self, n_steps):
+        """Record n_steps steps into a HIP graph (torch.cuda.CUDAGraph); each
+        ``replay()`` then advances every member by n_steps with one launch of
+        the graph.  The captured kernels use no memsets and the work lists do
+        not change, so replays repeat the eager launches exactly."""
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            _lib.check(self._L.boltx_ensemble_step(
+                self._n, self._members, ctypes.byref(self._tune), self.stepper,
+                _lib.ptr(self.f), _lib.ptr(self.u1), _lib.ptr(self.u2), _lib.ptr(self._ws),
+                int(n_steps), _lib.stream_handle()))
+        self._graph, self._graph_steps = g, int(n_steps)
+        return g
+
+    def replay(self):
+        """Run the captured steps (see capture)."""
+        self._graph.replay()
+        self.steps_taken += self._graph_steps
+
     def check_divergence(self):
         """timestep.py:28-38 for every member (one host sync)."""
         flag = self._ws[self._flag_off:self._flag_off + 4].view(torch.int32)
