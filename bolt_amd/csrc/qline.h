@@ -205,16 +205,22 @@ __device__ __forceinline__ void ql_march(const QLParams& q, int p, int seg, int 
       }
     }
   }
-  // periodic indices of the current row, the row after it and the next
-  // window row
+  // periodic indices of the current row, the row after it and the window
+  // row two rows ahead; the window row one ahead is already in flight
   int rcur = qm_mod(r0, n2);
   int rnx1 = rcur + 1 == n2 ? 0 : rcur + 1;
-  int rnxw = qm_mod(r0 + B + 5, n2);
+  int rnxw = qm_mod(r0 + B + 6, n2);
+  T yn[NR];
+  {
+    const T* uw = u + qm_mod(r0 + B + 5, n2) * rowP;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) yn[k] = __builtin_nontemporal_load(uw + i1[k]);
+  }
 
   for (int it = 0; it < q.L; ++it) {
-    // the next row's pointwise inputs and window row, one row ahead, so
-    // loads stay in flight across the stencil work
-    T yn[NR], wn[NR], unn[NR], tn1[NR], tn2[NR], tn3[NR];
+    // the next row's pointwise inputs one row ahead and the window row two
+    // rows ahead, so loads stay in flight across the stencil work
+    T ynn[NR], wn[NR], unn[NR], tn1[NR], tn2[NR], tn3[NR];
     {
       const T* uw = u + rnxw * rowP;
       const T* w1 = wv + rnx1 * rowP;
@@ -222,7 +228,7 @@ __device__ __forceinline__ void ql_march(const QLParams& q, int p, int seg, int 
       const long tq = (long)rnx1 * n1;
 #pragma unroll
       for (int k = 0; k < NR; ++k) {
-        yn[k] = __builtin_nontemporal_load(uw + i1[k]);
+        ynn[k] = __builtin_nontemporal_load(uw + i1[k]);
         wn[k] = w_is_u ? T(0) : __builtin_nontemporal_load(w1 + i1[k]);
         unn[k] = use_un ? __builtin_nontemporal_load(un1 + i1[k]) : T(0);
         tn1[k] = src ? t1p[tq + i1[k]] : T(0);
@@ -316,6 +322,7 @@ __device__ __forceinline__ void ql_march(const QLParams& q, int p, int seg, int 
 #pragma unroll
       for (int j = 0; j < 4; ++j) y[k][j] = y[k][j + 1];
       y[k][4] = yn[k];
+      yn[k] = ynn[k];
       if (do_q2) {
         ka[k] = kb[k];
         kb[k] = kc[k];
