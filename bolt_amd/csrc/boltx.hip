@@ -8,10 +8,14 @@
 // and its L2 holds the rows the stencil revisits.
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 #include "../../include/boltx.h"
 #include "fvm_stage.h"
@@ -425,10 +429,42 @@ static int launch_qc(const QCParams& qp, int rq, int rp, bool padv, hipStream_t 
 // ---- q1-line layout --------------------------------------------------------
 constexpr int ML_CHUNKS = 8;
 
-static int ql_segment(int n2) {
-  for (int L = 32; L > 1; L /= 2)
-    if (n2 % L == 0) return L;
-  return 1;
+// Number of q2 segments for ``groups`` workgroups per segment when ``slots``
+// workgroups are resident at once: segments of about 16 rows (measured at
+// 128^2 q x 48^2 p on an MI355X, 4 to 12 segments lie within 4 % of each
+// other, 8 is best in FP32 and within 1 % of best in FP64), more of them
+// (down to 4 rows) while the launch would fill fewer than two waves of
+// workgroups.  BOLTX_QL_SEGMENTS overrides the count (tuning sweeps).
+static int ql_segments(int n2, long groups, long slots) {
+  if (const char* e = getenv("BOLTX_QL_SEGMENTS")) {
+    const int v = atoi(e);
+    if (v >= 1) return v < n2 ? v : n2;
+  }
+  if (n2 <= 1) return 1;
+  int ns = n2 / 16 > 1 ? n2 / 16 : 1;
+  while (groups * ns < 2 * slots && n2 / (ns + 1) >= 4) ++ns;
+  return ns;
+}
+
+// Resident q1-line workgroups on the current device for the kernel picked
+// by (precision, n1, recon, mode, pd), cached per device and kernel.
+template <typename T>
+static long ql_slots(int n1, int recon, int mode, int pd) {
+  static std::mutex mu;
+  static std::map<std::array<int, 5>, long> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  const int nrc = n1 <= 64 ? 1 : n1 <= 128 ? 2 : 4;
+  const std::array<int, 5> key{dev, nrc, recon, mode, pd};
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    cus = 0;
+  const long v = (long)cus * ql_blocks_per_cu<T>(n1, recon, mode, pd);
+  cache[key] = v;
+  return v;
 }
 
 static size_t ql_table_bytes(const boltx_grid* g, int precision) {
@@ -483,8 +519,6 @@ static int ql_stage(const boltx_grid* g, const boltx_physics* ph, const boltx_st
   q.n1 = g->n_q1;
   q.n2 = g->n_q2;
   q.P = sp.P;
-  q.L = ql_segment(g->n_q2);
-  q.nseg = g->n_q2 / q.L;
   q.n_p1 = g->n_p1;
   q.n_p2 = g->n_p2;
   q.n_p3 = g->n_p3;
@@ -507,12 +541,14 @@ static int ql_stage(const boltx_grid* g, const boltx_physics* ph, const boltx_st
   q.p3s = g->p3_start;
   q.dp3 = g->dp3;
   q.tab = tab;
-  const int nb = (sp.P + 3) / 4 * q.nseg;
   const bool w_is_u = st->w == st->u;
   int mode = QL_GENERAL;
   if (w_is_u && st->a != 0.0) mode = QL_WU_COMB;
   else if (st->a == 0.0 && st->b == 1.0) mode = w_is_u ? QL_WU_PLAIN : QL_W_PLAIN;
   const int pd = sp.source ? sp.p_dim : 0;
+  const long groups = (sp.P + 3) / 4;
+  q.nseg = ql_segments(g->n_q2, groups, ql_slots<T>(g->n_q1, ph->recon_q, mode, pd));
+  const int nb = (int)(groups * q.nseg);
   ql_launch<T>(q, ph->recon_q, nb, mode, pd, s);
   if ((rc = launch_status("qline_kernel"))) return rc;
   if (st->mom_out) return ql_moments<T>(g, sp, st->out, st->mom_out, part, s);

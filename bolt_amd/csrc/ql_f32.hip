@@ -4,4 +4,5 @@
 
 namespace bx {
 template void ql_launch<float>(const QLParams&, int, int, int, int, hipStream_t);
+template int ql_blocks_per_cu<float>(int, int, int, int);
 }

@@ -4,7 +4,7 @@
 //
 // qline_kernel: a wave owns one p-cell and the whole periodic q1 line (NR
 // cells per lane, lane l holding q1 = l + 64 k), and marches along q2 over a
-// segment of L rows:
+// segment of rows:
 //  * q2: a 7-row register window per cell; the WENO indicator and the upwind
 //    state of each q2 face are computed once and carried to the next row;
 //  * q1: the row, its indicators and its face states go through a
@@ -26,7 +26,7 @@
 namespace bx {
 
 struct QLParams {
-  int n1, n2, P, L, nseg, n_p1, n_p2, n_p3;
+  int n1, n2, P, nseg, n_p1, n_p2, n_p3;  // nseg q2 segments of near-equal length
   const void* u;
   const void* w;
   const void* un;
@@ -139,7 +139,9 @@ __device__ __forceinline__ void ql_march(const QLParams& q, int p, int seg, int 
   const T C1 = T(centre(q.p1s, j1, q.dp1));
   const T C2 = T(centre(q.p2s, j2, q.dp2));
   const long rowP = (long)q.P * n1;  // elements per q2 row
-  const int r0 = seg * q.L;
+  // segment seg covers rows [seg*n2/nseg, (seg+1)*n2/nseg)
+  const int r0 = (int)((long)seg * n2 / q.nseg);
+  const int len = (int)((long)(seg + 1) * n2 / q.nseg) - r0;
   const T* u = (const T*)q.u + (long)p * n1;
   const T* wv = (const T*)q.w + (long)p * n1;
   const T* unv = (const T*)q.un + (long)p * n1;
@@ -217,7 +219,7 @@ __device__ __forceinline__ void ql_march(const QLParams& q, int p, int seg, int 
     for (int k = 0; k < NR; ++k) yn[k] = __builtin_nontemporal_load(uw + i1[k]);
   }
 
-  for (int it = 0; it < q.L; ++it) {
+  for (int it = 0; it < len; ++it) {
     // the next row's pointwise inputs one row ahead and the window row two
     // rows ahead, so loads stay in flight across the stencil work
     T ynn[NR], wn[NR], unn[NR], tn1[NR], tn2[NR], tn3[NR];

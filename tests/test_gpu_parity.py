@@ -222,6 +222,29 @@ QLINE_CASES = {
 }
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32-bulk"])
+def test_qline_segment_split_bit_identical(gpu, monkeypatch, precision):
+    """The q2 march is split into segments of near-equal length (any count,
+    13 rows into 1, 3, 5 or 13 segments here): every split gives the same
+    bits, since each segment recomputes its first face from the same window."""
+    from tests.cases import maxwellian_case
+    c = maxwellian_case(N_q1=64, N_q2=13, N_p1=8, N_p2=6, p_dim=2, modes=((1, 1),),
+                        amps=(6e-4,), phases=(0.4,), recon_q="weno5", recon_p="weno5",
+                        riemann="lax-friedrichs", stepper="SSP-RK3", tau=0.02,
+                        memory_layout="q1-lines", precision=precision)
+    c.dt = 5e-4
+    runs = {}
+    for ns in ("1", "3", "5", "13"):
+        monkeypatch.setenv("BOLTX_QL_SEGMENTS", ns)
+        nls = hip_solver(c)
+        assert nls._layout == 1
+        for _ in range(3):
+            nls.strang_timestep(c.dt)
+        runs[ns] = nls.f.clone()
+    for ns in ("3", "5", "13"):
+        assert torch.equal(runs[ns], runs["1"]), ns
+
+
 @pytest.mark.parametrize("name", sorted(QLINE_CASES))
 @pytest.mark.parametrize("rq", ["piecewise-constant", "minmod", "ppm", "weno5"])
 @pytest.mark.parametrize("stepper", ["RK2", "SSP-RK3"])
