@@ -71,6 +71,9 @@ __host__ __device__ inline T mc_slope(T ym, T y, T yp) {
 // ppm.py:57-98 on the five-cell stencil y0..y4 of the centre cell y2.
 template <typename T>
 __host__ __device__ inline void ppm_faces(T y0, T y1, T y2, T y3, T y4, T& left, T& right) {
+  // no contraction: a*b - c*d forms would otherwise round differently
+  // wherever the compiler fuses the other product
+#pragma clang fp contract(off)
   const T d0 = T(2) * (y1 - y0), d1 = T(2) * (y2 - y1);
   const T d2 = T(2) * (y3 - y2), d3 = T(2) * (y4 - y3);
   const T D1 = T(0.5) * (y2 - y0), D2 = T(0.5) * (y3 - y1), D3 = T(0.5) * (y4 - y2);

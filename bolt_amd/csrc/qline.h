@@ -5,9 +5,10 @@
 // qline_kernel: a wave owns one p-cell and the whole periodic q1 line (NR
 // cells per lane, lane l holding q1 = l + 64 k), and marches along q2 over a
 // segment of rows:
-//  * q2: a 7-row register window per cell; the WENO indicator and the upwind
-//    state of each q2 face are computed once and carried to the next row;
-//  * q1: the row, its indicators and its face states go through a
+//  * q2: a 6-row register window per cell (five stencil rows and the row
+//    loaded ahead); the WENO indicator and the upwind flux of each q2 face are
+//    computed once and carried to the next row;
+//  * q1: the row, its indicators and its face fluxes go through a
 //    wave-private LDS line (no workgroup barrier), so every q1 indicator and
 //    face is computed once;
 //  * one p per wave makes the upwind direction of both axes a scalar branch,
@@ -43,15 +44,28 @@ struct QLParams {
 // other two, one division per face); FP32 keeps 1/(16 beta^2) of the
 // sum-of-squares indicator (weno5.py:58-80), whose squares would underflow in
 // products.  Other reconstructions keep nothing.
+// Fused multiply-add of the march.  The march is compiled as several copies
+// (one per register rotation), and the compiler's choice of which product of
+// a*b + c*d to fuse depends on how often each product is reused, which
+// differs between copies; so the march's functions evaluate with contraction
+// off and fuse explicitly, and every row rounds the same way in every copy.
+__host__ __device__ inline double ql_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__host__ __device__ inline float ql_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 template <int R, typename T>
 __host__ __device__ inline T ql_kappa(T a, T b, T c) {
+#pragma clang fp contract(off)
   if constexpr (R != RC_WENO5) {
     return T(0);
   } else if constexpr (sizeof(T) == 8) {
     const double bt = weno_beta_literal(a, b, c);
     return bt * bt;
   } else {
-    const float bt = weno_beta_sos(a, b, c);
+    // 13/12 (a - 2b + c)^2 + 1/4 (a - 4b + 3c)^2 + eps (1 + |a| + |b| + |c|)
+    const float s1 = ql_fma(-2.0f, b, a) + c;
+    const float s2 = ql_fma(3.0f, c, ql_fma(-4.0f, b, a));
+    const float e = 1e-17f * (((1.0f + aabs(a)) + aabs(b)) + aabs(c));
+    const float bt = ql_fma((13.0f / 12.0f) * s1, s1, ql_fma(0.25f * s2, s2, e));
 #if defined(__HIP_DEVICE_COMPILE__)
     return __builtin_amdgcn_rcpf(16.0f * bt * bt);
 #else
@@ -68,16 +82,17 @@ __host__ __device__ inline T ql_kappa(T a, T b, T c) {
 // substencil coefficients (all exact in binary).
 template <int R, bool RIGHT, typename T>
 __host__ __device__ inline T ql_face(T y0, T y1, T y2, T y3, T y4, T ka, T kb, T kc) {
+#pragma clang fp contract(off)
   if constexpr (R == RC_WENO5) {
     T u1, u2, u3;
     if constexpr (RIGHT) {
-      u1 = T(0.375) * y0 - T(1.25) * y1 + T(1.875) * y2;
-      u2 = T(-1.25) * y1 + T(7.5) * y2 + T(3.75) * y3;
-      u3 = T(1.875) * y2 + T(3.75) * y3 - T(0.625) * y4;
+      u1 = ql_fma(T(1.875), y2, ql_fma(T(-1.25), y1, T(0.375) * y0));
+      u2 = ql_fma(T(3.75), y3, ql_fma(T(7.5), y2, T(-1.25) * y1));
+      u3 = ql_fma(T(-0.625), y4, ql_fma(T(3.75), y3, T(1.875) * y2));
     } else {
-      u1 = T(-0.625) * y0 + T(3.75) * y1 + T(1.875) * y2;
-      u2 = T(3.75) * y1 + T(7.5) * y2 - T(1.25) * y3;
-      u3 = T(1.875) * y2 - T(1.25) * y3 + T(0.375) * y4;
+      u1 = ql_fma(T(1.875), y2, ql_fma(T(3.75), y1, T(-0.625) * y0));
+      u2 = ql_fma(T(-1.25), y3, ql_fma(T(7.5), y2, T(3.75) * y1));
+      u3 = ql_fma(T(0.375), y4, ql_fma(T(-1.25), y3, T(1.875) * y2));
     }
     T w1, w2, w3;
     if constexpr (sizeof(T) == 8) {
@@ -89,12 +104,23 @@ __host__ __device__ inline T ql_face(T y0, T y1, T y2, T y3, T y4, T ka, T kb, T
       w2 = kb;
       w3 = kc;
     }
-    const T num = w1 * u1 + w2 * u2 + w3 * u3;
-    const T den = RIGHT ? (w1 + T(10) * w2) + T(5) * w3 : (T(5) * w1 + T(10) * w2) + w3;
+    const T num = ql_fma(w3, u3, ql_fma(w2, u2, w1 * u1));
+    const T den = RIGHT ? ql_fma(T(5), w3, ql_fma(T(10), w2, w1))
+                        : ql_fma(T(5), w1, ql_fma(T(10), w2, w3));
     return num * fast_rcp(den);
   } else {
     return face_value<R>(y0, y1, y2, y3, y4, RIGHT);
   }
+}
+
+// Upwind flux C * state through one face, rounded once (riemann.py:46-49 then
+// the C_q product of df_dt_fvm.py:56-85): each face's flux is formed once and
+// carried (q2) or shared through LDS (q1), so both cells of a face see the
+// same rounded value, whatever code copy computed it.
+template <typename T>
+__device__ __forceinline__ T ql_flux(T C, T state) {
+#pragma clang fp contract(off)
+  return C * state;
 }
 
 // Halo of the LDS q1 lines: cells -3..-1 and n1..n1+2 hold periodic copies,
@@ -128,9 +154,9 @@ enum QlMode : int { QL_WU_PLAIN = 0, QL_WU_COMB = 1, QL_W_PLAIN = 2, QL_GENERAL 
 // holds five rows y0..y4: rows r-2..r+2 (P2: the face r+1/2 is the right face
 // of row r) or r-1..r+3 (not P2: it is the left face of row r+1); either way
 // the face uses the window's centre y2 and the indicators of y1, y2, y3, and
-// the state of face r-1/2 is carried from the previous row.  Along q1 the row
-// goes through the wave's LDS line: values, then indicators, then the state
-// at each cell's lower face.
+// the flux through face r-1/2 is carried from the previous row.  Along q1 the
+// row goes through the wave's LDS line: values, then indicators, then the flux
+// through each cell's lower face.
 template <typename T, int R, int NR, bool P1, bool P2, int MODE, int PD>
 __device__ __forceinline__ void ql_march(const QLParams& q, int p, int seg, int lane, T* fl, T* kl,
                                          T* sl) {
@@ -173,7 +199,13 @@ __device__ __forceinline__ void ql_march(const QLParams& q, int p, int seg, int 
     if (!live[k]) i1[k] = 0;
   }
 
-  T y[NR][5], ka[NR], kb[NR], s2m[NR], wc[NR], unc[NR], tc1[NR], tc2[NR], tc3[NR];
+  // Register roles rotate instead of sliding: window row j of the row being
+  // updated lives in Y[(S + j) % 6], the indicators of its cells y1, y2 in
+  // K[S % 3], K[(S + 1) % 3], the carried lower q2 face flux and the row's
+  // pointwise inputs in slot S % 2, with S = it % 6.  Every load then lands in
+  // the register that consumes it, so no copy of a loaded value forces a wait
+  // on the loads (and stores) still in flight.
+  T Y[6][NR], K[3][NR], S2[2][NR], WC[2][NR], UNC[2][NR], TC1[2][NR], TC2[2][NR], TC3[2][NR];
   {
     // rows r0+B-1 .. r0+B+4: the lower face of row r0 from the first five,
     // then the window
@@ -187,42 +219,53 @@ __device__ __forceinline__ void ql_march(const QLParams& q, int p, int seg, int 
       rr = rr + 1 == n2 ? 0 : rr + 1;
       row = u + rr * rowP;
 #pragma unroll
-      for (int k = 0; k < NR; ++k) y[k][j] = row[i1[k]];
+      for (int k = 0; k < NR; ++k) Y[j][k] = row[i1[k]];
     }
     const int rs0 = qm_mod(r0, n2);
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
-      wc[k] = w_is_u ? T(0) : wv[rs0 * rowP + i1[k]];
-      unc[k] = use_un ? unv[rs0 * rowP + i1[k]] : T(0);
-      tc1[k] = src ? t1p[(long)rs0 * n1 + i1[k]] : T(0);
-      tc2[k] = src2 ? t2p[(long)rs0 * n1 + i1[k]] : T(0);
-      tc3[k] = src3 ? t3p[(long)rs0 * n1 + i1[k]] : T(0);
-      s2m[k] = T(0);
-      ka[k] = kb[k] = T(0);
+      WC[0][k] = w_is_u ? T(0) : wv[rs0 * rowP + i1[k]];
+      UNC[0][k] = use_un ? unv[rs0 * rowP + i1[k]] : T(0);
+      TC1[0][k] = src ? t1p[(long)rs0 * n1 + i1[k]] : T(0);
+      TC2[0][k] = src2 ? t2p[(long)rs0 * n1 + i1[k]] : T(0);
+      TC3[0][k] = src3 ? t3p[(long)rs0 * n1 + i1[k]] : T(0);
+      S2[0][k] = T(0);
+      K[0][k] = K[1][k] = T(0);
       if (do_q2) {
-        const T k0 = ql_kappa<R, T>(z[k], y[k][0], y[k][1]);
-        ka[k] = ql_kappa<R, T>(y[k][0], y[k][1], y[k][2]);
-        kb[k] = ql_kappa<R, T>(y[k][1], y[k][2], y[k][3]);
-        s2m[k] = ql_face<R, P2, T>(z[k], y[k][0], y[k][1], y[k][2], y[k][3], k0, ka[k], kb[k]);
+        const T k0 = ql_kappa<R, T>(z[k], Y[0][k], Y[1][k]);
+        K[0][k] = ql_kappa<R, T>(Y[0][k], Y[1][k], Y[2][k]);
+        K[1][k] = ql_kappa<R, T>(Y[1][k], Y[2][k], Y[3][k]);
+        S2[0][k] = ql_flux(C2, ql_face<R, P2, T>(z[k], Y[0][k], Y[1][k], Y[2][k], Y[3][k], k0,
+                                                 K[0][k], K[1][k]));
       }
     }
   }
-  // periodic indices of the current row, the row after it and the window
-  // row two rows ahead; the window row one ahead is already in flight
+  // LDS addresses each lane cell is written to: the cell and its periodic
+  // halo copy (or the cell again); a lane past the line writes both to a
+  // private slot past the halo, so every write is unconditional
+  int a1[NR], h1[NR];
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    a1[k] = live[k] ? i1[k] : NR * 64 + QL_HALO + lane;
+    h1[k] = !live[k] ? a1[k]
+            : i1[k] < QL_HALO ? i1[k] + n1
+            : i1[k] >= n1 - QL_HALO ? i1[k] - n1 : i1[k];
+  }
+  // periodic indices of the current row, the row after it and the window row
+  // that enters the window at the next row
   int rcur = qm_mod(r0, n2);
   int rnx1 = rcur + 1 == n2 ? 0 : rcur + 1;
-  int rnxw = qm_mod(r0 + B + 6, n2);
-  T yn[NR];
-  {
-    const T* uw = u + qm_mod(r0 + B + 5, n2) * rowP;
-#pragma unroll
-    for (int k = 0; k < NR; ++k) yn[k] = __builtin_nontemporal_load(uw + i1[k]);
-  }
+  int rnxw = qm_mod(r0 + B + 5, n2);
 
-  for (int it = 0; it < len; ++it) {
-    // the next row's pointwise inputs one row ahead and the window row two
-    // rows ahead, so loads stay in flight across the stencil work
-    T ynn[NR], wn[NR], unn[NR], tn1[NR], tn2[NR], tn3[NR];
+  auto row = [&](auto rot) {
+#pragma clang fp contract(off)
+    constexpr int S = decltype(rot)::value;
+    constexpr int y0 = S % 6, y1 = (S + 1) % 6, y2 = (S + 2) % 6, y3 = (S + 3) % 6,
+                  y4 = (S + 4) % 6, yl = (S + 5) % 6;
+    constexpr int ka = S % 3, kb = (S + 1) % 3, kc = (S + 2) % 3;
+    constexpr int cu = S % 2, nx = (S + 1) % 2;
+    // the window row of the next row and the next row's pointwise inputs,
+    // one row ahead, so the loads stay in flight across the stencil work
     {
       const T* uw = u + rnxw * rowP;
       const T* w1 = wv + rnx1 * rowP;
@@ -230,34 +273,36 @@ __device__ __forceinline__ void ql_march(const QLParams& q, int p, int seg, int 
       const long tq = (long)rnx1 * n1;
 #pragma unroll
       for (int k = 0; k < NR; ++k) {
-        ynn[k] = __builtin_nontemporal_load(uw + i1[k]);
-        wn[k] = w_is_u ? T(0) : __builtin_nontemporal_load(w1 + i1[k]);
-        unn[k] = use_un ? __builtin_nontemporal_load(un1 + i1[k]) : T(0);
-        tn1[k] = src ? t1p[tq + i1[k]] : T(0);
-        tn2[k] = src2 ? t2p[tq + i1[k]] : T(0);
-        tn3[k] = src3 ? t3p[tq + i1[k]] : T(0);
+        Y[yl][k] = __builtin_nontemporal_load(uw + i1[k]);
+        WC[nx][k] = w_is_u ? T(0) : __builtin_nontemporal_load(w1 + i1[k]);
+        UNC[nx][k] = use_un ? __builtin_nontemporal_load(un1 + i1[k]) : T(0);
+        TC1[nx][k] = src ? t1p[tq + i1[k]] : T(0);
+        TC2[nx][k] = src2 ? t2p[tq + i1[k]] : T(0);
+        TC3[nx][k] = src3 ? t3p[tq + i1[k]] : T(0);
       }
     }
     T d[NR], fc[NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
-      fc[k] = P2 ? y[k][2] : y[k][1];
+      fc[k] = P2 ? Y[y2][k] : Y[y1][k];
       d[k] = T(0);
     }
-    T kc[NR];
     if (do_q2) {
 #pragma unroll
       for (int k = 0; k < NR; ++k) {
-        kc[k] = ql_kappa<R, T>(y[k][2], y[k][3], y[k][4]);
-        const T s2p = ql_face<R, P2, T>(y[k][0], y[k][1], y[k][2], y[k][3], y[k][4], ka[k],
-                                        kb[k], kc[k]);
-        d[k] = (C2 * s2p - C2 * s2m[k]) * tinv2;
-        s2m[k] = s2p;
+        K[kc][k] = ql_kappa<R, T>(Y[y2][k], Y[y3][k], Y[y4][k]);
+        const T f2p = ql_flux(C2, ql_face<R, P2, T>(Y[y0][k], Y[y1][k], Y[y2][k], Y[y3][k],
+                                                    Y[y4][k], K[ka][k], K[kb][k], K[kc][k]));
+        d[k] = (f2p - S2[cu][k]) * tinv2;
+        S2[nx][k] = f2p;
       }
     }
     if (do_q1) {
 #pragma unroll
-      for (int k = 0; k < NR; ++k) ql_put(fl, i1[k], live[k], n1, fc[k]);
+      for (int k = 0; k < NR; ++k) {
+        fl[a1[k]] = fc[k];
+        fl[h1[k]] = fc[k];
+      }
       __builtin_amdgcn_wave_barrier();
       // P1: cells i-3, i-2, i-1, i+1; otherwise i-2, i-1, i+1, i+2
       T va[NR], vb[NR], vc[NR], vd[NR], kown[NR];
@@ -270,10 +315,13 @@ __device__ __forceinline__ void ql_march(const QLParams& q, int p, int seg, int 
         vd[k] = line[P1 ? 1 : 2];
         // neighbours i-1 and i+1 of the cell: (vc, vd) for P1, (vb, vc) otherwise
         kown[k] = P1 ? ql_kappa<R, T>(vc[k], fc[k], vd[k]) : ql_kappa<R, T>(vb[k], fc[k], vc[k]);
-        ql_put(kl, i1[k], live[k], n1, kown[k]);
+        if constexpr (R == RC_WENO5) {
+          kl[a1[k]] = kown[k];
+          kl[h1[k]] = kown[k];
+        }
       }
       __builtin_amdgcn_wave_barrier();
-      T s[NR];
+      T sf[NR];
 #pragma unroll
       for (int k = 0; k < NR; ++k) {
         if constexpr (P1) {
@@ -283,7 +331,7 @@ __device__ __forceinline__ void ql_march(const QLParams& q, int p, int seg, int 
             k2 = kl[i1[k] - 2];
             k1 = kl[i1[k] - 1];
           }
-          s[k] = ql_face<R, true, T>(va[k], vb[k], vc[k], fc[k], vd[k], k2, k1, kown[k]);
+          sf[k] = ql_face<R, true, T>(va[k], vb[k], vc[k], fc[k], vd[k], k2, k1, kown[k]);
         } else {
           // left face of cell i: window i-2..i+2, indicators of i-1, i, i+1
           T k1 = T(0), kp = T(0);
@@ -291,53 +339,58 @@ __device__ __forceinline__ void ql_march(const QLParams& q, int p, int seg, int 
             k1 = kl[i1[k] - 1];
             kp = kl[i1[k] + 1];
           }
-          s[k] = ql_face<R, false, T>(va[k], vb[k], fc[k], vc[k], vd[k], k1, kown[k], kp);
+          sf[k] = ql_face<R, false, T>(va[k], vb[k], fc[k], vc[k], vd[k], k1, kown[k], kp);
         }
-        ql_put(sl, i1[k], live[k], n1, s[k]);
+        sf[k] = ql_flux(C1, sf[k]);
+        sl[a1[k]] = sf[k];
+        sl[h1[k]] = sf[k];
       }
       __builtin_amdgcn_wave_barrier();
 #pragma unroll
       for (int k = 0; k < NR; ++k) {
-        const T su = sl[i1[k] + 1];  // lower face of cell i+1
-        d[k] = -(C1 * su - C1 * s[k]) * tinv1 - d[k];
+        const T fu = sl[i1[k] + 1];  // flux through the lower face of cell i+1
+        d[k] = ql_fma(sf[k] - fu, tinv1, -d[k]);  // -(fu - sf) / dq1 - d
       }
       __builtin_amdgcn_wave_barrier();
     } else {
 #pragma unroll
       for (int k = 0; k < NR; ++k) d[k] = -d[k];
     }
-    // source, RK combination, store; slide the window
+    // source, RK combination, store
     T* orow = outv + rcur * rowP;
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
       const T f = fc[k];
       T dk = d[k];
       if (src) {
-        T fmb = tc1[k];
-        if (src2) fmb = fmb * tc2[k];
-        if (src3) fmb = fmb * tc3[k];
-        dk = dk + (fmb - f) * tinvtau;
+        T fmb = TC1[cu][k];
+        if (src2) fmb = fmb * TC2[cu][k];
+        if (src3) fmb = fmb * TC3[cu][k];
+        dk = ql_fma(fmb - f, tinvtau, dk);
       }
-      const T tv = (w_is_u ? f : wc[k]) + dk * tcdt;
-      const T x = use_un ? ta * unc[k] + tb * tv : (plain ? tv : tb * tv);
+      const T tv = ql_fma(dk, tcdt, w_is_u ? f : WC[cu][k]);
+      const T x = use_un ? ql_fma(ta, UNC[cu][k], tb * tv) : (plain ? tv : tb * tv);
       if (live[k]) __builtin_nontemporal_store(x, orow + i1[k]);
-#pragma unroll
-      for (int j = 0; j < 4; ++j) y[k][j] = y[k][j + 1];
-      y[k][4] = yn[k];
-      yn[k] = ynn[k];
-      if (do_q2) {
-        ka[k] = kb[k];
-        kb[k] = kc[k];
-      }
-      wc[k] = wn[k];
-      unc[k] = unn[k];
-      tc1[k] = tn1[k];
-      tc2[k] = tn2[k];
-      tc3[k] = tn3[k];
     }
     rcur = rnx1;
     rnx1 = rnx1 + 1 == n2 ? 0 : rnx1 + 1;
     rnxw = rnxw + 1 == n2 ? 0 : rnxw + 1;
+  };
+  using std::integral_constant;
+  // six rows per pass, so the rotation is back at 0 on the loop's back edge
+  for (int it = 0; it < len;) {
+    row(integral_constant<int, 0>());
+    if (++it == len) break;
+    row(integral_constant<int, 1>());
+    if (++it == len) break;
+    row(integral_constant<int, 2>());
+    if (++it == len) break;
+    row(integral_constant<int, 3>());
+    if (++it == len) break;
+    row(integral_constant<int, 4>());
+    if (++it == len) break;
+    row(integral_constant<int, 5>());
+    ++it;
   }
 }
 
@@ -349,9 +402,9 @@ template <typename T, int R, int NR, int MODE, int PD>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void qline_kernel(QLParams q) {
   // wave-private q1 lines with three halo cells on each side: row values,
   // q1 indicator quantities and lower-face states
-  __shared__ T fl[4][NR * 64 + 2 * QL_HALO];
-  __shared__ T kl[4][NR * 64 + 2 * QL_HALO];
-  __shared__ T sl[4][NR * 64 + 2 * QL_HALO];
+  __shared__ T fl[4][NR * 64 + 2 * QL_HALO + 64];
+  __shared__ T kl[4][NR * 64 + 2 * QL_HALO + 64];
+  __shared__ T sl[4][NR * 64 + 2 * QL_HALO + 64];
   // the wave index is uniform across the wave: keep it (and all that derives
   // from it: p-cell, velocities, row pointers) in scalar registers
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
