@@ -13,13 +13,14 @@ RECON = {"piecewise-constant": 0, "minmod": 1, "ppm": 2, "weno5": 3}
 RIEMANN = {"upwind-flux": 0, "lax-friedrichs": 1}
 PRECISION = {"fp64": 0, "fp32-bulk": 1}
 LAYOUT = {"reference": 0, "q1-lines": 1}
+BC = {"periodic": 0, "none": 0, "mirror": 1, "dirichlet": 2, "mirror+dirichlet": 3}
 STEPPER = {"RK2": 0, "SSP-RK3": 1}
 
 EXPORTS = ("boltx_fvm_stage", "boltx_moments", "boltx_workspace_size", "boltx_workspace_init",
            "boltx_currents", "boltx_fdtd_step", "boltx_yee_to_cell_centered",
            "boltx_check_finite", "boltx_adjoint_stage", "boltx_adjoint_workspace_size",
            "boltx_ensemble_layout", "boltx_ensemble_workspace_size", "boltx_ensemble_map",
-           "boltx_ensemble_init", "boltx_ensemble_step",
+           "boltx_ensemble_init", "boltx_ensemble_step", "boltx_fill_ghosts",
            "boltx_version", "boltx_last_error")
 MOM_ROWS = 12
 
@@ -31,7 +32,8 @@ class Grid(ctypes.Structure):
                 ("dq1", ctypes.c_double), ("dq2", ctypes.c_double),
                 ("dp1", ctypes.c_double), ("dp2", ctypes.c_double), ("dp3", ctypes.c_double),
                 ("p1_start", ctypes.c_double), ("p2_start", ctypes.c_double),
-                ("p3_start", ctypes.c_double)]
+                ("p3_start", ctypes.c_double), ("n_ghost", ctypes.c_int32),
+                ("bc", ctypes.c_int32 * 4)]
 
 
 class Physics(ctypes.Structure):
@@ -105,6 +107,7 @@ def load(path=LIB_PATH):
                                        ctypes.POINTER(ctypes.c_size_t)]
     lib.boltx_ensemble_init.argtypes = [i32, em, tu, vp, vp, vp]
     lib.boltx_ensemble_step.argtypes = [i32, em, tu, i32, vp, vp, vp, vp, i32, vp]
+    lib.boltx_fill_ghosts.argtypes = [g, ph, vp, ctypes.POINTER(vp), vp]
     for name in EXPORTS[:-2]:
         getattr(lib, name).restype = ctypes.c_int
     lib.boltx_workspace_size.restype = ctypes.c_size_t

@@ -31,6 +31,8 @@ def _check_solver(s):
         raise NotImplementedError("the adjoint runs in plain FP64 (precision='fp64')")
     if s._fields_on:
         raise NotImplementedError('the adjoint needs fields off')
+    if s._G:
+        raise NotImplementedError('the adjoint needs periodic q boundaries')
 
 
 def interior_to_internal(s, x):

@@ -291,6 +291,62 @@ __global__ void check_finite_kernel(const double* n, int Q, int* flag) {
   if (t < Q && !isfinite(n[t])) flag[0] = 1;
 }
 
+// Ghost q-rows of the non-periodic sides (boundaries.py:147-311, 415-577 after
+// the zero fill of a ghosted DMDA axis): one thread per (ghost row, p).
+struct GhostParams {
+  int n_q1, n_q2, n_p1, n_p2, P, Q, ng, gh1, rows;
+  int bc[4];
+  double p1s, dp1, p2s, dp2;
+  const void* dir[4];
+};
+
+template <typename T>
+__global__ void fill_ghosts_kernel(GhostParams g, T* f) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.rows * g.P) return;
+  const int r = (int)(t / g.P), p = (int)(t % g.P);
+  // side, ghost cell c and the cell index along the boundary
+  int side, c, io, local;
+  const int b1 = g.gh1 ? 2 * g.ng * g.n_q2 : 0;
+  if (r < b1) {
+    side = r < g.ng * g.n_q2 ? 0 : 1;
+    local = r - side * g.ng * g.n_q2;
+    c = local / g.n_q2;
+    io = local % g.n_q2;
+  } else {
+    const int rr = r - b1;
+    side = rr < g.ng * g.n_q1 ? 2 : 3;
+    local = rr - (side - 2) * g.ng * g.n_q1;
+    c = local / g.n_q1;
+    io = local % g.n_q1;
+  }
+  const int kind = g.bc[side];
+  const int j1 = p % g.n_p1, j2 = (p / g.n_p1) % g.n_p2;
+  T v = T(0);  // a ghosted DMDA boundary zone is never written: zero
+  if (kind == BOLTX_BC_MIRROR || kind == BOLTX_BC_MIRROR_DIRICHLET) {
+    // mirror image of interior cell ng-1-c (left/bottom) or n-1-c (right/top),
+    // normal velocity reversed
+    int qs, ps;
+    if (side < 2) {
+      const int i1 = side == 0 ? g.ng - 1 - c : g.n_q1 - 1 - c;
+      qs = i1 + g.n_q1 * io;
+      ps = p + (g.n_p1 - 1 - 2 * j1);
+    } else {
+      const int i2 = side == 2 ? g.ng - 1 - c : g.n_q2 - 1 - c;
+      qs = io + g.n_q1 * i2;
+      ps = p + (g.n_p2 - 1 - 2 * j2) * g.n_p1;
+    }
+    v = f[(long)qs * g.P + ps];
+  }
+  if (kind == BOLTX_BC_DIRICHLET || kind == BOLTX_BC_MIRROR_DIRICHLET) {
+    // inflow: C_q1 = p1 > 0 on the left, < 0 on the right; C_q2 = p2 likewise
+    const double C = side < 2 ? centre(g.p1s, j1, g.dp1) : centre(g.p2s, j2, g.dp2);
+    const bool in = (side % 2 == 0) ? C > 0.0 : C < 0.0;
+    if (in) v = ((const T*)g.dir[side])[(long)local * g.P + p];
+  }
+  f[(long)(g.Q + r) * g.P + p] = v;
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -306,11 +362,34 @@ int check_grid(const boltx_grid* g) {
   if (g->layout == BOLTX_LAYOUT_Q1_LINES && g->n_q1 > 256)
     return fail(BOLTX_EUNSUPPORTED, "the q1-line layout holds at most 256 q1 cells (got %d)",
                 g->n_q1);
+  for (int k = 0; k < 4; ++k)
+    if (g->bc[k] < BOLTX_BC_PERIODIC || g->bc[k] > BOLTX_BC_MIRROR_DIRICHLET)
+      return fail(BOLTX_EINVAL, "bc[%d] = %d", k, g->bc[k]);
+  for (int a = 0; a < 2; ++a)
+    if ((g->bc[2 * a] == BOLTX_BC_PERIODIC) != (g->bc[2 * a + 1] == BOLTX_BC_PERIODIC))
+      return fail(BOLTX_EINVAL, "a periodic q axis is periodic on both sides");
+  const bool gh1 = g->bc[0] != BOLTX_BC_PERIODIC, gh2 = g->bc[2] != BOLTX_BC_PERIODIC;
+  if (gh1 || gh2) {
+    if (g->layout != BOLTX_LAYOUT_REFERENCE)
+      return fail(BOLTX_EUNSUPPORTED, "non-periodic q boundaries need the reference layout");
+    if (g->n_ghost < 1 || g->n_ghost > 3)
+      return fail(BOLTX_EINVAL, "n_ghost = %d (1..3 with non-periodic boundaries)", g->n_ghost);
+    if ((gh1 && g->n_q1 < g->n_ghost) || (gh2 && g->n_q2 < g->n_ghost))
+      return fail(BOLTX_EINVAL, "a non-periodic q axis needs at least n_ghost cells");
+  }
   const long P = (long)g->n_p1 * g->n_p2 * g->n_p3;
   const long Q = (long)g->n_q1 * g->n_q2;
   if (P * Q > (1L << 40)) return fail(BOLTX_EINVAL, "grid too large");
   if (Q > 0x7fffffff) return fail(BOLTX_EINVAL, "too many q-cells");
   return BOLTX_OK;
+}
+
+// Ghost q-rows of the non-periodic axes (0 when q is periodic).
+static long ghost_rows(const boltx_grid* g) {
+  long n = 0;
+  if (g->bc[0] != BOLTX_BC_PERIODIC) n += 2L * g->n_ghost * g->n_q2;
+  if (g->bc[2] != BOLTX_BC_PERIODIC) n += 2L * g->n_ghost * g->n_q1;
+  return n;
 }
 
 static StageParams base_params(const boltx_grid* g) {
@@ -335,6 +414,9 @@ static StageParams base_params(const boltx_grid* g) {
   sp.inv_dp2 = 1.0 / g->dp2;
   sp.inv_dp3 = 1.0 / g->dp3;
   sp.measure = g->dp3 * g->dp2 * g->dp1;
+  sp.gh1 = g->bc[0] != BOLTX_BC_PERIODIC;
+  sp.gh2 = g->bc[2] != BOLTX_BC_PERIODIC;
+  sp.ng = (sp.gh1 || sp.gh2) ? g->n_ghost : 0;
   return sp;
 }
 
@@ -591,6 +673,10 @@ extern "C" int boltx_fvm_stage(const boltx_grid* grid, const boltx_physics* phys
   const int need = recon_reach(phys->recon_q) + 1;
   if ((grid->n_q1 > 1 && grid->n_q1 < need) || (grid->n_q2 > 1 && grid->n_q2 < need))
     return fail(BOLTX_EINVAL, "q-grid smaller than the stencil reach");
+  if (ghost_rows(grid) > 0 && grid->n_ghost < need)
+    return fail(BOLTX_EINVAL, "n_ghost = %d is below the stencil reach %d", grid->n_ghost, need);
+  if (ghost_rows(grid) > 0 && phys->p_advection)
+    return fail(BOLTX_EUNSUPPORTED, "non-periodic q boundaries with p-advection");
   StageParams sp = base_params(grid);
   sp.riemann_q = phys->riemann_q;
   sp.riemann_p = phys->riemann_p;
@@ -659,6 +745,45 @@ extern "C" int boltx_moments(const boltx_grid* grid, const boltx_physics* phys, 
   else
     hipLaunchKernelGGL(moments_kernel<float>, dim3(sp.Q), dim3(kBlock), 0, s, sp);
   return launch_status("moments_kernel");
+}
+
+extern "C" int boltx_fill_ghosts(const boltx_grid* grid, const boltx_physics* phys, void* f,
+                                 const void* const dirichlet[4], void* stream) {
+  int rc = check_grid(grid);
+  if (rc) return rc;
+  if ((rc = check_physics(phys))) return rc;
+  const long rows = ghost_rows(grid);
+  if (rows == 0) return BOLTX_OK;
+  if (!f) return fail(BOLTX_EINVAL, "f is NULL");
+  GhostParams g;
+  memset(&g, 0, sizeof(g));
+  g.n_q1 = grid->n_q1;
+  g.n_q2 = grid->n_q2;
+  g.n_p1 = grid->n_p1;
+  g.n_p2 = grid->n_p2;
+  g.P = grid->n_p1 * grid->n_p2 * grid->n_p3;
+  g.Q = grid->n_q1 * grid->n_q2;
+  g.ng = grid->n_ghost;
+  g.gh1 = grid->bc[0] != BOLTX_BC_PERIODIC;
+  g.rows = (int)rows;
+  g.p1s = grid->p1_start;
+  g.dp1 = grid->dp1;
+  g.p2s = grid->p2_start;
+  g.dp2 = grid->dp2;
+  for (int k = 0; k < 4; ++k) {
+    g.bc[k] = grid->bc[k];
+    g.dir[k] = dirichlet ? dirichlet[k] : nullptr;
+    if ((g.bc[k] == BOLTX_BC_DIRICHLET || g.bc[k] == BOLTX_BC_MIRROR_DIRICHLET) && !g.dir[k])
+      return fail(BOLTX_EINVAL, "dirichlet values of side %d are NULL", k);
+  }
+  const long n = rows * g.P;
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  hipStream_t s = (hipStream_t)stream;
+  if (phys->precision == BOLTX_FP64)
+    hipLaunchKernelGGL(fill_ghosts_kernel<double>, dim3(nb), dim3(256), 0, s, g, (double*)f);
+  else
+    hipLaunchKernelGGL(fill_ghosts_kernel<float>, dim3(nb), dim3(256), 0, s, g, (float*)f);
+  return launch_status("fill_ghosts_kernel");
 }
 
 extern "C" size_t boltx_workspace_size(const boltx_grid* grid, const boltx_physics* phys) {
@@ -771,6 +896,10 @@ extern "C" int boltx_adjoint_stage(const boltx_grid* grid, const boltx_physics* 
   const int need = recon_reach(phys->recon_q) + 1;
   if ((grid->n_q1 > 1 && grid->n_q1 < need) || (grid->n_q2 > 1 && grid->n_q2 < need))
     return fail(BOLTX_EINVAL, "q-grid smaller than the stencil reach");
+  if (ghost_rows(grid) > 0 && grid->n_ghost < need)
+    return fail(BOLTX_EINVAL, "n_ghost = %d is below the stencil reach %d", grid->n_ghost, need);
+  if (ghost_rows(grid) > 0 && phys->p_advection)
+    return fail(BOLTX_EUNSUPPORTED, "non-periodic q boundaries with p-advection");
   const AdjParams ap = adj_params(grid, phys, adj);
   hipStream_t s = (hipStream_t)stream;
   const long Q = (long)ap.n1 * ap.n2;

@@ -29,6 +29,9 @@ struct StageParams {
   const double* fp1;  // [6][Q] fields seen by C_p1
   const double* fp2;
   const double* fp3;
+  // non-periodic q axes: ghost q-rows follow the Q interior rows of every
+  // f-sized array (left, right, bottom, top blocks of ng rows per q-line)
+  int ng, gh1, gh2;
 };
 
 // Per-q-cell state the stage needs besides f: the stencil neighbours of the
@@ -36,8 +39,8 @@ struct StageParams {
 template <typename T>
 struct QCell {
   int q;
-  int nb1[7];  // q-cells i1-3 .. i1+3 (same i2), periodic
-  int nb2[7];  // q-cells i2-3 .. i2+3 (same i1), periodic
+  int nb1[7];  // rows of q-cells i1-3 .. i1+3 (same i2): periodic images or ghost rows
+  int nb2[7];  // rows of q-cells i2-3 .. i2+3 (same i1)
   double pref; // n (m / 2 pi k T)^(p_dim/2)
   const T* e1; // exp(-m (v1-u1)^2 / 2kT) per p1 index
   const T* e2;
@@ -45,15 +48,36 @@ struct QCell {
   double E[3], B[3][3];  // fields for C_p1 (row 0), C_p2 (row 1), C_p3 (row 2)
 };
 
+// Row of q-cell (j1, i2) where j1 may lie outside [0, n_q1): the periodic
+// image, or the ghost row of a non-periodic axis (a stencil never reaches
+// past the ng ghost cells; beyond them the index is clamped).
+__host__ __device__ inline int q_row_1(const StageParams& sp, int j1, int i2) {
+  if (j1 >= 0 && j1 < sp.n_q1) return j1 + sp.n_q1 * i2;
+  if (!sp.gh1) return ((j1 % sp.n_q1 + sp.n_q1) % sp.n_q1) + sp.n_q1 * i2;
+  const bool right = j1 >= sp.n_q1;
+  int c = right ? j1 - sp.n_q1 : j1 + sp.ng;
+  c = c < 0 ? 0 : (c >= sp.ng ? sp.ng - 1 : c);
+  return sp.Q + (right ? sp.ng * sp.n_q2 : 0) + c * sp.n_q2 + i2;
+}
+
+// The same along q2 for q-cell (i1, j2).
+__host__ __device__ inline int q_row_2(const StageParams& sp, int i1, int j2) {
+  if (j2 >= 0 && j2 < sp.n_q2) return i1 + sp.n_q1 * j2;
+  if (!sp.gh2) return i1 + sp.n_q1 * ((j2 % sp.n_q2 + sp.n_q2) % sp.n_q2);
+  const bool top = j2 >= sp.n_q2;
+  int c = top ? j2 - sp.n_q2 : j2 + sp.ng;
+  c = c < 0 ? 0 : (c >= sp.ng ? sp.ng - 1 : c);
+  return sp.Q + (sp.gh1 ? 2 * sp.ng * sp.n_q2 : 0) + (top ? sp.ng * sp.n_q1 : 0) +
+         c * sp.n_q1 + i1;
+}
+
 template <typename T>
 __host__ __device__ inline void qcell_neighbours(const StageParams& sp, int q, QCell<T>& c) {
   const int i1 = q % sp.n_q1, i2 = q / sp.n_q1;
   c.q = q;
   for (int k = 0; k < 7; ++k) {
-    const int j1 = ((i1 + k - 3) % sp.n_q1 + sp.n_q1) % sp.n_q1;
-    const int j2 = ((i2 + k - 3) % sp.n_q2 + sp.n_q2) % sp.n_q2;
-    c.nb1[k] = j1 + sp.n_q1 * i2;
-    c.nb2[k] = i1 + sp.n_q1 * j2;
+    c.nb1[k] = q_row_1(sp, i1 + k - 3, i2);
+    c.nb2[k] = q_row_2(sp, i1, i2 + k - 3);
   }
 }
 

@@ -85,7 +85,7 @@ def load_distribution_function(solver, file_name):
     x = torch.from_numpy(v).to(s.device).reshape(s.N_q2, s.N_q1, s._P)
     if s._layout != 0:
         x = x.permute(0, 2, 1)
-    s._f = x.contiguous().to(s._dtype)
+    s._f = s._store(x.to(s._dtype))
     s._update_moments()
 
 

@@ -11,8 +11,11 @@ p-grids are (N_p, N_s, 1, 1).
 
 Internally f is stored without ghost zones as a contiguous [N_q2][N_q1][N_p]
 array (periodic q is handled by modular indexing in the kernels), FP64 or -- with
-``params.precision = 'fp32-bulk'`` -- FP32 with FP64 moments.  There is no CPU
-path: construction fails if the HIP library or the GPU is missing.
+``params.precision = 'fp32-bulk'`` -- FP32 with FP64 moments.  Non-periodic q
+boundaries ('mirror', 'dirichlet', 'mirror+dirichlet'; boundaries.py) keep
+their ghost zones as extra q-rows after the interior, refilled on the device
+before every RHS evaluation.  There is no CPU path: construction fails if the
+HIP library or the GPU is missing.
 
 Additions to the reference's parameters (all optional):
   ``riemann_solver_in_q/p``: 'upwind-flux' (reference) or 'lax-friedrichs';
@@ -58,16 +61,22 @@ class nonlinear_solver(object):
         self.performance_test_flag = performance_test_flag
         if self.N_species != 1:
             raise NotImplementedError('the fused kernels support one species')
-        # nonlinear_solver.py:177-249: periodic / none boundaries only
-        for side in ('in_q1_left', 'in_q1_right', 'in_q2_bottom', 'in_q2_top'):
-            if getattr(bc, side) not in ('periodic', 'none'):
+        # nonlinear_solver.py:177-249 and boundaries.py:415-577
+        sides = ('in_q1_left', 'in_q1_right', 'in_q2_bottom', 'in_q2_top')
+        for side in sides:
+            if getattr(bc, side) == 'shearing-box':
+                raise NotImplementedError('shearing-box boundaries are not provided')
+            if getattr(bc, side) not in _lib.BC:
                 raise NotImplementedError('Unavailable/Invalid boundary condition')
         for a, b in (('in_q1_left', 'in_q1_right'), ('in_q2_bottom', 'in_q2_top')):
-            if getattr(bc, a) != getattr(bc, b):
-                raise Exception('Periodic boundary conditions need to be applied to '
-                                'both the boundaries of a particular axis')
+            for kind in ('periodic', 'none'):
+                if (getattr(bc, a) == kind) != (getattr(bc, b) == kind):
+                    raise Exception('%s boundary conditions need to be applied to both the '
+                                    'boundaries of a particular axis' % kind.capitalize())
         self.N_g1 = 0 if bc.in_q1_left == 'none' else self.N_ghost
         self.N_g2 = 0 if bc.in_q2_bottom == 'none' else self.N_ghost
+        self._bc = [_lib.BC[getattr(bc, side)] for side in sides]
+        self._ghosted = (self._bc[0] != 0, self._bc[2] != 0)
         if bc.in_q1_left == 'none' and self.N_q1 != 1:
             raise NotImplementedError("'none' boundaries are supported on singleton axes")
         if bc.in_q2_bottom == 'none' and self.N_q2 != 1:
@@ -90,7 +99,8 @@ class nonlinear_solver(object):
         self._grid = _lib.Grid(self.N_q1, self.N_q2, self.N_p1, self.N_p2, self.N_p3, 1,
                                self._layout, float(self.dq1), float(self.dq2), self._dp[0], self._dp[1],
                                self._dp[2], float(ps.p1_start[0]), float(ps.p2_start[0]),
-                               float(ps.p3_start[0]))
+                               float(ps.p3_start[0]), int(self.N_ghost),
+                               (ctypes.c_int32 * 4)(*self._bc))
         self._phys = _lib.Physics(self._recon_q, self._recon_p, self._riemann_q,
                                   self._riemann_p, int(self._source_on), int(self._p_adv),
                                   int(getattr(p, 'p_dim', 1)), self._precision,
@@ -99,6 +109,12 @@ class nonlinear_solver(object):
                                   float(self._tau))
         Q, P = self.N_q1 * self.N_q2, self.N_p1 * self.N_p2 * self.N_p3
         self._Q, self._P = Q, P
+        # ghost q-rows of the non-periodic axes after the Q interior rows
+        self._G = (2 * self.N_ghost * self.N_q2 if self._ghosted[0] else 0) + \
+                  (2 * self.N_ghost * self.N_q1 if self._ghosted[1] else 0)
+        self._fshape = (self.N_q2, self.N_q1, P) if self._layout == 0 else \
+            (self.N_q2, P, self.N_q1)
+        self._dir_zero = None
         self._bufs = {}
         self._mom = torch.zeros(_lib.MOM_ROWS, Q, dtype=F64, device=self.device)
         nws = int(self._L.boltx_workspace_size(ctypes.byref(self._grid), ctypes.byref(self._phys)))
@@ -144,12 +160,26 @@ class nonlinear_solver(object):
         # reference byte order [q2][q1][p]
         lay = getattr(p, 'memory_layout', 'auto')
         fields = bool(getattr(p, 'fields_enabled', False))
+        ghosted = any(self._ghosted)
         if lay == 'auto':
-            lay = 'q1-lines' if (not fields and ps.N_q1 in (64, 128, 256)) else 'reference'
+            lay = 'q1-lines' if (not fields and not ghosted and ps.N_q1 in (64, 128, 256)) \
+                else 'reference'
         if lay not in _lib.LAYOUT:
             raise NotImplementedError("memory_layout must be 'auto', 'reference' or 'q1-lines'")
         if lay == 'q1-lines' and (fields or ps.N_q1 > 256):
             raise NotImplementedError('the q1-line layout needs fields off and N_q1 <= 256')
+        if lay == 'q1-lines' and ghosted:
+            raise NotImplementedError('non-periodic q boundaries need the reference layout')
+        if ghosted:
+            if fields:
+                raise NotImplementedError('non-periodic q boundaries are provided with fields off')
+            if getattr(p, 'p_space_grid', 'cartesian') != 'cartesian' and \
+                    any(k in (1, 3) for k in self._bc):
+                raise NotImplementedError('Unsupported coordinate system in p_space')
+            need = {'piecewise-constant': 1, 'minmod': 2}.get(p.reconstruction_method_in_q, 3)
+            if self.N_ghost < need:
+                raise NotImplementedError('N_ghost = %d is below the reach of %s'
+                                          % (self.N_ghost, p.reconstruction_method_in_q))
         self._layout = _lib.LAYOUT[lay]
         if self._stepper not in ('RK2', 'SSP-RK3'):
             raise NotImplementedError('time_stepper_fvm must be RK2 or SSP-RK3')
@@ -240,6 +270,23 @@ class nonlinear_solver(object):
         i2 = (torch.arange(-self.N_g2, self.N_q2 + self.N_g2, device=a.device)) % self.N_q2
         return a[i1][:, i2].reshape(1, 1, len(i1), len(i2))
 
+    def _alloc_f(self, dtype=None):
+        """An f-sized array in the storage layout whose memory continues with
+        the G ghost q-rows of the non-periodic axes (the view covers the
+        interior; the kernels address the rows after it)."""
+        dtype = dtype or self._dtype
+        n = self._Q * self._P
+        store = torch.empty((self._Q + self._G) * self._P, dtype=dtype, device=self.device)
+        return store[:n].view(self._fshape)
+
+    def _store(self, x):
+        """x (storage layout) as an array with room for the ghost rows."""
+        if self._G == 0:
+            return x.contiguous().to(self._dtype)
+        out = self._alloc_f()
+        out.copy_(x)
+        return out
+
     def _to_internal(self, f_ref):
         """(P, S, Q1g, Q2g) -> the contiguous storage layout ([N_q2][N_q1][P]
         or [N_q2][P][N_q1]) in the storage dtype."""
@@ -247,20 +294,108 @@ class nonlinear_solver(object):
         f_ref = torch.broadcast_to(f_ref, (self._P, 1) + tuple(self.q1_center.shape[2:]))
         inner = self._q_interior(f_ref)[:, 0]  # (P, N_q1, N_q2)
         perm = (2, 1, 0) if self._layout == 0 else (2, 0, 1)
-        return inner.permute(*perm).contiguous().to(self._dtype)
+        return self._store(inner.permute(*perm))
+
+    def _padded_rows(self):
+        """Row of the [Q + G] storage behind every padded (Q1g, Q2g) cell: the
+        periodic image, a ghost row, or -1 for the corners (no stencil reads
+        them; they read as zero)."""
+        if getattr(self, '_rows_cache', None) is not None:
+            return self._rows_cache
+        n1, n2, g = self.N_q1, self.N_q2, self.N_ghost
+        i1 = torch.arange(-self.N_g1, n1 + self.N_g1).reshape(-1, 1)
+        i2 = torch.arange(-self.N_g2, n2 + self.N_g2).reshape(1, -1)
+        in1 = (i1 >= 0) & (i1 < n1)
+        in2 = (i2 >= 0) & (i2 < n2)
+        j1, j2 = i1 % n1, i2 % n2
+        rows = j1 + n1 * j2
+        Q = self._Q
+        # a periodic axis's ghost cells are its periodic images, also beside
+        # the other axis's ghost rows
+        if self._ghosted[0]:
+            c = torch.where(i1 < 0, i1 + g, i1 - n1)
+            r = Q + torch.where(i1 < 0, 0, g * n2) + c * n2 + j2
+            rows = torch.where(~in1 & (in2 | (not self._ghosted[1])), r, rows)
+        if self._ghosted[1]:
+            c = torch.where(i2 < 0, i2 + g, i2 - n2)
+            off = Q + (2 * g * n2 if self._ghosted[0] else 0)
+            r = off + torch.where(i2 < 0, 0, g * n1) + c * n1 + j1
+            rows = torch.where((in1 | (not self._ghosted[0])) & ~in2, r, rows)
+        if any(self._ghosted):
+            outside = (~in1 if self._ghosted[0] else torch.zeros_like(in1)) & \
+                      (~in2 if self._ghosted[1] else torch.zeros_like(in2))
+            rows = torch.where(outside, -1, rows)
+        self._rows_cache = rows.to(self.device)
+        return self._rows_cache
 
     def _to_reference(self, f_int):
-        """Storage layout -> (P, 1, Q1g, Q2g) with periodic ghost zones."""
-        i1 = torch.arange(-self.N_g1, self.N_q1 + self.N_g1, device=f_int.device) % self.N_q1
-        i2 = torch.arange(-self.N_g2, self.N_q2 + self.N_g2, device=f_int.device) % self.N_q2
-        if self._layout == 0:
-            return f_int[i2][:, i1].permute(2, 1, 0).unsqueeze(1)
-        return f_int[i2][:, :, i1].permute(1, 2, 0).unsqueeze(1)
+        """Storage layout -> (P, 1, Q1g, Q2g): periodic ghost zones from the
+        interior, non-periodic ones from the ghost rows (as last filled)."""
+        if self._G == 0:
+            i1 = torch.arange(-self.N_g1, self.N_q1 + self.N_g1, device=f_int.device) % self.N_q1
+            i2 = torch.arange(-self.N_g2, self.N_q2 + self.N_g2, device=f_int.device) % self.N_q2
+            if self._layout == 0:
+                return f_int[i2][:, i1].permute(2, 1, 0).unsqueeze(1)
+            return f_int[i2][:, :, i1].permute(1, 2, 0).unsqueeze(1)
+        n = (self._Q + self._G) * self._P
+        full = torch.as_strided(f_int, (self._Q + self._G, self._P), (self._P, 1))
+        assert full.untyped_storage().nbytes() >= n * f_int.element_size()
+        rows = self._padded_rows()
+        out = full[rows.clamp(min=0)]  # (Q1g, Q2g, P)
+        out = torch.where((rows >= 0).unsqueeze(-1), out, torch.zeros((), dtype=out.dtype,
+                                                                      device=out.device))
+        return out.permute(2, 0, 1).unsqueeze(1)
+
+    def _fill_ghosts(self, u):
+        """communicate_f + apply_bcs_f (communicate.py:8-38, boundaries.py:
+        415-577) on the ghost rows of u before a RHS evaluation.  Dirichlet
+        sides call the user's f_left / f_right / f_bottom / f_top(f, t, q1,
+        q2, p1, p2, p3, params) with f in the reference layout, its ghost
+        zones zero or mirrored (the user's values are not in yet), and take
+        their values where the characteristic enters."""
+        if self._G == 0:
+            return
+        dirs = (ctypes.c_void_p * 4)()
+        keep = []
+        dsides = [k for k in range(4) if self._bc[k] in (2, 3)]
+        if dsides:
+            if self._dir_zero is None:
+                n = self.N_ghost * max(self.N_q1, self.N_q2) * self._P
+                self._dir_zero = torch.zeros(n, dtype=self._dtype, device=self.device)
+            for k in dsides:
+                dirs[k] = self._dir_zero.data_ptr()
+            _lib.check(self._L.boltx_fill_ghosts(ctypes.byref(self._grid), ctypes.byref(self._phys),
+                                                 _lib.ptr(u), dirs, _lib.stream_handle()))
+            fpad = self._to_reference(u).to(F64)
+            bcm = self.boundary_conditions
+            args = (fpad, self.time_elapsed, self.q1_center, self.q2_center, self.p1_center,
+                    self.p2_center, self.p3_center, self.params)
+            g1, g2, g = self.N_g1, self.N_g2, self.N_ghost
+            full = (self._P, 1) + tuple(self.q1_center.shape[2:])
+            for k in dsides:
+                fn = getattr(bcm, ('f_left', 'f_right', 'f_bottom', 'f_top')[k])
+                val = torch.broadcast_to(torch.as_tensor(fn(*args), dtype=F64,
+                                                         device=self.device), full)[:, 0]
+                if k == 0:
+                    blk = val[:, :g, g2:g2 + self.N_q2]                  # (P, c, i2)
+                elif k == 1:
+                    blk = val[:, g1 + self.N_q1:g1 + self.N_q1 + g, g2:g2 + self.N_q2]
+                elif k == 2:
+                    blk = val[:, g1:g1 + self.N_q1, :g].transpose(1, 2)  # (P, c, i1)
+                else:
+                    blk = val[:, g1:g1 + self.N_q1, g2 + self.N_q2:g2 + self.N_q2 + g].transpose(1, 2)
+                blk = blk.permute(1, 2, 0).contiguous().to(self._dtype)  # [c][i_other][P]
+                keep.append(blk)
+                dirs[k] = blk.data_ptr()
+        _lib.check(self._L.boltx_fill_ghosts(ctypes.byref(self._grid), ctypes.byref(self._phys),
+                                             _lib.ptr(u), dirs, _lib.stream_handle()))
 
     @property
     def f(self):
         """The distribution function in the reference's (P, S, Q1g, Q2g)
-        layout, ghost zones filled periodically (a copy)."""
+        layout (a copy): periodic ghost zones filled from the interior,
+        non-periodic ones as the boundary conditions fill them."""
+        self._fill_ghosts(self._f)
         return self._to_reference(self._f)
 
     @f.setter
@@ -284,11 +419,15 @@ class nonlinear_solver(object):
                                          _lib.ptr(self._ws), _lib.stream_handle()))
 
     def _buf(self, name, dtype=None, shape=None):
+        """A cached work array; f-sized ones (no shape given) have room for
+        the ghost rows."""
         dtype = dtype or self._dtype
+        fsized = shape is None
         shape = shape or tuple(self._f.shape)
         t = self._bufs.get(name)
         if t is None or t.dtype != dtype or tuple(t.shape) != tuple(shape):
-            t = torch.empty(shape, dtype=dtype, device=self.device)
+            t = self._alloc_f(dtype) if fsized else torch.empty(shape, dtype=dtype,
+                                                                 device=self.device)
             self._bufs[name] = t
         return t
 
@@ -297,6 +436,7 @@ class nonlinear_solver(object):
         """One RK stage, out = a*un + b*(w + cdt*L(u)) (df_dt_fvm.py:33-339
         inside fvm_operator.py:22-33)."""
         F = [None, None, None]
+        self._fill_ghosts(u)  # fvm_operator.py:22-33: ghost refresh before each RHS
         if self._p_adv:
             fs = self.fields_solver
             if self._es:  # df_dt_fvm.py:235-242: rho of the stage input

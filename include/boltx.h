@@ -12,8 +12,8 @@
  *   - return 0 on success, a negative boltx_status otherwise; the message is
  *     in boltx_last_error() (thread-local).
  *
- * Layout of a distribution function: interior q-cells only (periodic q is
- * handled by modular indexing, so there are no ghost zones), P = n_p1*n_p2*n_p3
+ * Layout of a distribution function: interior q-cells (periodic q is handled
+ * by modular indexing, so a periodic axis has no ghost zones), P = n_p1*n_p2*n_p3
  * with p-index i1 + n_p1*(i2 + n_p2*i3) (p1 fastest), and either
  *   BOLTX_LAYOUT_REFERENCE: C-order [n_q2][n_q1][P], the byte order of the
  *     reference's (P, S, Q1, Q2) column-major af.Array for S = 1 without its
@@ -21,6 +21,13 @@
  *   BOLTX_LAYOUT_Q1_LINES: C-order [n_q2][P][n_q1] (q1 fastest, n_q1 <= 256),
  *     used for fields-off problems: every q1 line of one p-cell is contiguous.
  * FP64 storage is double, FP32-bulk storage is float.
+ * Non-periodic q boundaries (grid->bc, reference layout only): every f-sized
+ * array holds G = 2*n_ghost*(n_q2 if q1 is non-periodic) + 2*n_ghost*(n_q1 if
+ * q2 is non-periodic) ghost q-rows of P values after its Q interior rows:
+ * left [c][i2] (q1 = c - n_ghost), right [c][i2] (q1 = n_q1 + c), bottom
+ * [c][i1] (q2 = c - n_ghost), top [c][i1] (q2 = n_q2 + c), c = 0..n_ghost-1,
+ * the blocks of periodic axes left out.  boltx_fill_ghosts fills them; the
+ * stage kernels read them as the stencil neighbours beyond the boundary.
  * A moment block is an FP64 array [BOLTX_MOM_ROWS][Q] (Q = n_q1*n_q2): rows
  * 0-4 are density, mom_v1_bulk, mom_v2_bulk, mom_v3_bulk and energy, each
  * already multiplied by dp3*dp2*dp1; rows 5-11 are the BGK parameters derived
@@ -50,6 +57,14 @@ enum boltx_recon {
 enum boltx_riemann { BOLTX_RIEMANN_UPWIND = 0, BOLTX_RIEMANN_LLF = 1 };
 enum boltx_precision { BOLTX_FP64 = 0, BOLTX_FP32_BULK = 1 };
 enum boltx_layout { BOLTX_LAYOUT_REFERENCE = 0, BOLTX_LAYOUT_Q1_LINES = 1 };
+/* q boundary of one side (boundaries.py:415-577; 'none' is a singleton axis
+ * marked periodic: it has no neighbours). */
+enum boltx_bc {
+  BOLTX_BC_PERIODIC = 0,
+  BOLTX_BC_MIRROR = 1,
+  BOLTX_BC_DIRICHLET = 2,
+  BOLTX_BC_MIRROR_DIRICHLET = 3
+};
 enum boltx_status {
   BOLTX_OK = 0,
   BOLTX_EINVAL = -1,
@@ -58,13 +73,16 @@ enum boltx_status {
 };
 
 typedef struct boltx_grid {
-  int32_t n_q1, n_q2;       /* interior q-cells, periodic */
+  int32_t n_q1, n_q2;       /* interior q-cells */
   int32_t n_p1, n_p2, n_p3; /* p-cells per axis */
   int32_t n_species;        /* 1 */
   int32_t layout;           /* enum boltx_layout of every f-sized array */
   double dq1, dq2;
   double dp1, dp2, dp3;
   double p1_start, p2_start, p3_start;
+  int32_t n_ghost;          /* ghost width of the non-periodic q axes */
+  int32_t bc[4];            /* enum boltx_bc of the left, right, bottom, top sides;
+                               a periodic axis is periodic on both sides */
 } boltx_grid;
 
 typedef struct boltx_physics {
@@ -108,6 +126,21 @@ int boltx_fvm_stage(const boltx_grid *grid, const boltx_physics *phys,
  * parameters (collision_operator.py:102-116). */
 int boltx_moments(const boltx_grid *grid, const boltx_physics *phys, const void *f,
                   double *mom, void *workspace, void *stream);
+
+/* Fill the ghost q-rows of f (layout above) for the non-periodic sides, as
+ * communicate_f + apply_bcs_f leave them before a RHS evaluation
+ * (communicate.py:8-38 with the 'ghosted' DMDA axes of
+ * nonlinear_solver.py:177-201, then boundaries.py:415-577): start from zero
+ * (the DMDA never writes a ghosted boundary zone); mirror sides copy the
+ * mirror image of the first n_ghost interior cells with the normal velocity
+ * reversed (p1 for left/right, p2 for bottom/top; apply_mirror_bcs_f_cartesian,
+ * boundaries.py:227-311); dirichlet sides then take dirichlet[side] where the
+ * characteristic enters (C_q1 = p1 > 0 on the left, < 0 on the right, C_q2 = p2
+ * likewise; apply_dirichlet_bcs_f, boundaries.py:147-225).  dirichlet[side]
+ * holds that side's ghost block ([c][i_other][P], storage type) of the user's
+ * f_<side> values, and may be NULL for non-dirichlet sides. */
+int boltx_fill_ghosts(const boltx_grid *grid, const boltx_physics *phys, void *f,
+                      const void *const dirichlet[4], void *stream);
 
 /* Bytes of device workspace boltx_fvm_stage and boltx_moments use for this
  * problem (required in the q1-line layout, optional in the reference layout,

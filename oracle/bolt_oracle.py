@@ -27,6 +27,12 @@ riemann (upwind / LLF)   LLF == upwind to rounding for a linear flux; C=0 ->
                          zero flux; select picks the right state at C == 0
 df_dt_fvm                periodic conservation (sum of df/dt = 0), constant
                          state is stationary, DFT eigenvalue of a sine mode
+ghosted_fill /           mirror walls conserve mass exactly and keep a
+apply_bcs_f              mirror-symmetric state symmetric; piecewise-constant
+                         + upwind with dirichlet inflow equals the first-order
+                         upwind scheme with inflow values, written from its
+                         definition; a uniform state equal to the inflow
+                         value is stationary
 compute_moments          the reference KAT ``test_compute_moments.py:76-144``
                          (drifting Maxwellian, 1e-13)
 BGK / f0                 C[f_MB] ~ 0, conservation of n, nu, E, relaxation
@@ -632,12 +638,30 @@ def periodic_fill(x, N_g1, N_g2):
     return x
 
 
+def ghosted_fill(x, N_g1, N_g2, periodic1, periodic2):
+    """communicate.py:8-38 when an axis is not periodic: the DMDA is created
+    with a 'ghosted' boundary on that axis (nonlinear_solver.py:177-201,
+    265-278), globalToLocal fills only the periodic axes, and the local
+    vector's entries in a ghosted boundary zone are never written, so they
+    stay at the zero they were created with; petsc_local_array_to_af copies
+    them into f (af_petsc_conversion.py:23-41)."""
+    x = periodic_fill(x, N_g1 if periodic1 else 0, N_g2 if periodic2 else 0)
+    x = x.clone()
+    if not periodic1 and N_g1:
+        x[:, :, :N_g1] = 0
+        x[:, :, x.shape[2] - N_g1:] = 0
+    if not periodic2 and N_g2:
+        x[:, :, :, :N_g2] = 0
+        x[:, :, :, x.shape[3] - N_g2:] = 0
+    return x
+
+
 # ---------------------------------------------------------------------------
 # Solver: nonlinear_solver.py + finite_volume/{df_dt_fvm,fvm_operator}.py
 # ---------------------------------------------------------------------------
 class OracleSolver:
-    """nonlinear_solver.py:55-521 for the FVM path (periodic / none q-BCs,
-    one rank).  ``system`` is any object with the attributes of
+    """nonlinear_solver.py:55-521 for the FVM path (periodic, none, mirror,
+    dirichlet and mirror+dirichlet q-BCs on a cartesian p-grid, one rank).  ``system`` is any object with the attributes of
     physical_system (domain sizes, params, initial_conditions, C_q, C_p,
     source, moments, boundary_conditions)."""
 
@@ -653,8 +677,14 @@ class OracleSolver:
         self.N_g1 = 0 if bc.in_q1_left == "none" else self.N_ghost  # :197-201
         self.N_g2 = 0 if bc.in_q2_bottom == "none" else self.N_ghost
         for side in ("in_q1_left", "in_q1_right", "in_q2_bottom", "in_q2_top"):
-            if getattr(bc, side) not in ("periodic", "none"):
-                raise NotImplementedError("oracle restates periodic / none q-BCs only")
+            if getattr(bc, side) not in ("periodic", "none", "mirror", "dirichlet",
+                                         "mirror+dirichlet"):
+                raise NotImplementedError("oracle restates periodic, none, mirror and "
+                                          "dirichlet q-BCs")
+        # nonlinear_solver.py:187-195: periodic DMDA axes, else 'ghosted'
+        self.periodic1 = bc.in_q1_left == "periodic"
+        self.periodic2 = bc.in_q2_bottom == "periodic"
+        self.boundary_conditions = bc
         # nonlinear_solver.py:133-145: species lists -> (1, S) arrays
         S = self.N_species
         self.mass = torch.tensor(list(p.mass), dtype=F64).reshape(1, S, 1, 1)
@@ -713,9 +743,88 @@ class OracleSolver:
                        S * self.f.shape[2] * self.f.shape[3])
 
     def _communicate_f(self):
-        """communicate.py:8-38 (periodic fill); apply_bcs_f is a no-op for
-        periodic / none (boundaries.py:454-457)."""
-        self.f = periodic_fill(self.f, self.N_g1, self.N_g2)
+        """communicate.py:8-38 then apply_bcs_f (boundaries.py:415-577), the
+        ghost refresh before every RHS evaluation (fvm_operator.py:22-33)."""
+        self.f = ghosted_fill(self.f, self.N_g1, self.N_g2, self.periodic1, self.periodic2)
+        self.apply_bcs_f()
+
+    # -- boundaries.py ------------------------------------------------------
+    def _mirror(self, boundary):
+        """apply_mirror_bcs_f_cartesian (boundaries.py:227-311): the ghost
+        zone is the mirror image of the first N_g interior cells, with the
+        velocity component normal to the wall reversed (p1 for left/right,
+        p2 for bottom/top)."""
+        g = self.N_ghost
+        f = self.f.clone()
+        if boundary == "left":
+            f[:, :, :g] = torch.flip(f[:, :, g:2 * g], (2,))
+            f[:, :, :g] = self._convert_to_q_expanded(
+                torch.flip(self._convert_to_p_expanded(f), (0,)))[:, :, :g]
+        elif boundary == "right":
+            n = f.shape[2]
+            f[:, :, n - g:] = torch.flip(f[:, :, n - 2 * g:n - g], (2,))
+            f[:, :, n - g:] = self._convert_to_q_expanded(
+                torch.flip(self._convert_to_p_expanded(f), (0,)))[:, :, n - g:]
+        elif boundary == "bottom":
+            f[:, :, :, :g] = torch.flip(f[:, :, :, g:2 * g], (3,))
+            f[:, :, :, :g] = self._convert_to_q_expanded(
+                torch.flip(self._convert_to_p_expanded(f), (1,)))[:, :, :, :g]
+        else:
+            n = f.shape[3]
+            f[:, :, :, n - g:] = torch.flip(f[:, :, :, n - 2 * g:n - g], (3,))
+            f[:, :, :, n - g:] = self._convert_to_q_expanded(
+                torch.flip(self._convert_to_p_expanded(f), (1,)))[:, :, :, n - g:]
+        self.f = f
+
+    def _dirichlet(self, boundary):
+        """apply_dirichlet_bcs_f (boundaries.py:147-225): the user's
+        f_<boundary>(f, t, q1, q2, p1, p2, p3, params) replaces the ghost
+        zone where the characteristic enters the domain (C_q1 > 0 on the left,
+        < 0 on the right; C_q2 likewise on bottom / top); elsewhere the ghost
+        zone keeps what it holds."""
+        g = self.N_ghost
+        p = self.params
+        C_q1, C_q2 = self._C_q(self.time_elapsed, self.q1_center, self.q2_center,
+                               self.p1_center, self.p2_center, self.p3_center, p)
+        full = self.f.shape
+        C_q1 = torch.broadcast_to(as4(C_q1), full)
+        C_q2 = torch.broadcast_to(as4(C_q2), full)
+        bcm = self.boundary_conditions
+        args = (self.f, self.time_elapsed, self.q1_center, self.q2_center,
+                self.p1_center, self.p2_center, self.p3_center, p)
+        f = self.f.clone()
+        if boundary == "left":
+            val = torch.where(C_q1 > 0, torch.broadcast_to(as4(bcm.f_left(*args)), full), f)
+            f[:, :, :g] = val[:, :, :g]
+        elif boundary == "right":
+            n = full[2]
+            val = torch.where(C_q1 < 0, torch.broadcast_to(as4(bcm.f_right(*args)), full), f)
+            f[:, :, n - g:] = val[:, :, n - g:]
+        elif boundary == "bottom":
+            val = torch.where(C_q2 > 0, torch.broadcast_to(as4(bcm.f_bottom(*args)), full), f)
+            f[:, :, :, :g] = val[:, :, :, :g]
+        else:
+            n = full[3]
+            val = torch.where(C_q2 < 0, torch.broadcast_to(as4(bcm.f_top(*args)), full), f)
+            f[:, :, :, n - g:] = val[:, :, :, n - g:]
+        self.f = f
+
+    def apply_bcs_f(self):
+        """boundaries.py:415-577 on one rank: left, right, bottom, top in
+        that order; periodic and none need nothing; 'mirror+dirichlet'
+        mirrors first.  Cartesian p-grid only (p_space_grid, :437-450)."""
+        bc = self.boundary_conditions
+        for boundary, name in (("left", "in_q1_left"), ("right", "in_q1_right"),
+                               ("bottom", "in_q2_bottom"), ("top", "in_q2_top")):
+            kind = getattr(bc, name)
+            if kind in ("periodic", "none"):
+                continue
+            if kind in ("mirror", "mirror+dirichlet"):
+                if getattr(self.params, "p_space_grid", "cartesian") != "cartesian":
+                    raise NotImplementedError("Unsupported coordinate system in p_space")
+                self._mirror(boundary)
+            if kind in ("dirichlet", "mirror+dirichlet"):
+                self._dirichlet(boundary)
 
     def compute_moments(self, moment_name, f=None):
         """compute_moments.py:7-59 (cartesian p-grid)."""

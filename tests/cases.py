@@ -244,3 +244,53 @@ def case_es(seed=0, N_q1=32, N_q2=1, N_p=24, **over):
     c = maxwellian_case(**kw)
     c.dt, c.steps = 1e-3, 100
     return c
+
+
+def drifting_maxwellian(p1, p2, p3, p_dim, n, u1, u2, T=1.0):
+    """n (2 pi T)^(-p_dim/2) exp(-((p1-u1)^2 + (p2-u2)^2 [+ p3^2]) / 2T)."""
+    e = (p1 - u1) ** 2
+    if p_dim >= 2:
+        e = e + (p2 - u2) ** 2
+    if p_dim >= 3:
+        e = e + p3 ** 2
+    return n * (2 * math.pi * T) ** (-p_dim / 2) * torch.exp(-e / (2 * T))
+
+
+def case_bc(*, left="mirror", right="mirror", bottom="periodic", top="periodic",
+            N_q1=8, N_q2=6, N_p1=6, N_p2=6, p_max=6.0, p_dim=2, N_ghost=3,
+            recon_q="weno5", riemann="upwind-flux", stepper="RK2", source=False,
+            tau=0.05, precision="fp64", drift=(0.7, -0.4), bump=0.3, inflow=None,
+            dt=2e-3):
+    """Non-periodic q boundaries (boundaries.py): a drifting Maxwellian with a
+    Gaussian density bump centred in [0,1)^2 (width 0.2, amplitude ``bump``),
+    mean velocity ``drift``, and per-side q-BCs; dirichlet sides inject a
+    drifting Maxwellian (density, u1, u2) = ``inflow`` or (1.2, 0.5, 0.3)
+    through f_left / f_right / f_bottom / f_top(f, t, q1, q2, p1, p2, p3,
+    params).  1-D problems (N_q2 = 1) take bottom = top = 'none'."""
+    if N_q2 == 1:
+        bottom = top = "none"
+    c = maxwellian_case(N_q1=N_q1, N_q2=N_q2, N_p1=N_p1, N_p2=N_p2 if p_dim >= 2 else 1,
+                        N_ghost=N_ghost, p_max=p_max, p_dim=p_dim, recon_q=recon_q,
+                        recon_p=recon_q, riemann=riemann, stepper=stepper, tau=tau,
+                        source=source, precision=precision, memory_layout="auto")
+    n_in, u1_in, u2_in = inflow or (1.2, 0.5, 0.3)
+
+    def f_side(f, t, q1, q2, p1, p2, p3, params):
+        return drifting_maxwellian(p1, p2, p3, params.p_dim, n_in, u1_in, u2_in) \
+            * (q1 * 0 + 1) * (q2 * 0 + 1)
+
+    c.boundary_conditions = _module(
+        "boundary_conditions", in_q1_left=left, in_q1_right=right,
+        in_q2_bottom=bottom, in_q2_top=top, f_left=f_side, f_right=f_side,
+        f_bottom=f_side, f_top=f_side)
+    c.params.drift, c.params.bump = tuple(drift), bump
+
+    def initialize_f(q1, q2, p1, p2, p3, params):
+        r2 = (q1 - 0.5) ** 2 + ((q2 - 0.5) ** 2 if N_q2 > 1 else 0 * q2)
+        n = 1.0 + params.bump * torch.exp(-r2 / (2 * 0.2 ** 2))
+        return drifting_maxwellian(p1, p2, p3, params.p_dim, n, params.drift[0],
+                                   params.drift[1])
+
+    c.initialize.initialize_f = initialize_f
+    c.dt, c.steps = dt, 4
+    return c
