@@ -219,16 +219,25 @@ QLINE_CASES = {
                     phases=(1.1,)),
     "2d3v_off_64x4": dict(N_q1=64, N_q2=4, N_p1=6, N_p2=6, N_p3=6, p_dim=3, p_max=8.0,
                           modes=((1, 1),), amps=(5e-3,), phases=(0.2,)),
+    # two (128 cells) and four (256 cells) cells per lane
+    "2d2v_128x6": dict(N_q1=128, N_q2=6, N_p1=6, N_p2=6, p_dim=2, modes=((2, 1),),
+                       amps=(9e-4,), phases=(0.5,)),
+    "1d2v_256": dict(N_q1=256, N_q2=1, N_p1=6, N_p2=4, p_dim=2, modes=((3, 0),), amps=(7e-4,),
+                     phases=(2.3,)),
+    "2d1v_256x5": dict(N_q1=256, N_q2=5, N_p1=8, p_dim=1, modes=((1, 1),), amps=(5e-4,),
+                       phases=(0.9,)),
 }
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32-bulk"])
-def test_qline_segment_split_bit_identical(gpu, monkeypatch, precision):
+@pytest.mark.parametrize("n1", [64, 128])
+def test_qline_segment_split_bit_identical(gpu, monkeypatch, precision, n1):
     """The q2 march is split into segments of near-equal length (any count,
     13 rows into 1, 3, 5 or 13 segments here): every split gives the same
-    bits, since each segment recomputes its first face from the same window."""
+    bits, since each segment recomputes its first face from the same window
+    (one cell per lane at 64 cells, two at 128)."""
     from tests.cases import maxwellian_case
-    c = maxwellian_case(N_q1=64, N_q2=13, N_p1=8, N_p2=6, p_dim=2, modes=((1, 1),),
+    c = maxwellian_case(N_q1=n1, N_q2=13, N_p1=8, N_p2=6, p_dim=2, modes=((1, 1),),
                         amps=(6e-4,), phases=(0.4,), recon_q="weno5", recon_p="weno5",
                         riemann="lax-friedrichs", stepper="SSP-RK3", tau=0.02,
                         memory_layout="q1-lines", precision=precision)
