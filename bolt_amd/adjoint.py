@@ -33,6 +33,8 @@ def _check_solver(s):
         raise NotImplementedError('the adjoint needs fields off')
     if s._G:
         raise NotImplementedError('the adjoint needs periodic q boundaries')
+    if s.N_species != 1:
+        raise NotImplementedError('the adjoint is provided for one species')
 
 
 def interior_to_internal(s, x):

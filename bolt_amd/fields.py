@@ -42,7 +42,7 @@ class fields_solver(object):
             self._initialize_magnetic_fields()
             self._yee_to_cc(cc)
         elif init in ('fft', 'fft + user-defined magnetic fields'):  # fields.py:465-488
-            rho = solver._charge * solver._mom[0]
+            rho = solver._rho()
             self._poisson(rho, cc)
             self._cc_to_yee_E(cc)
             if init != 'fft':
@@ -103,7 +103,7 @@ class fields_solver(object):
         cc[1] = (torch.fft.ifft2(-2j * math.pi * k2 * phi).real / eps).reshape(-1)
 
     def compute_electrostatic_fields(self, rho):
-        """fields.py:559-566 (one species: no sum needed)."""
+        """fields.py:559-566; rho is [Q], already summed over species."""
         self._poisson(rho, self._cc)
 
     def _cc_to_yee_E(self, cc):

@@ -71,22 +71,30 @@ def moment_names(solver):
 
 
 def dump_distribution_function(solver, file_name):
-    """dump.py:121-168: f's interior in natural order, to file_name + '.bin'."""
+    """dump.py:121-168: f's interior in natural order (dof = p + P s fastest,
+    then q1, then q2), to file_name + '.bin'."""
     s = solver
-    f = s._f if s._layout == 0 else s._f.permute(0, 2, 1)  # -> [N_q2][N_q1][P]
-    write_vec(file_name + '.bin', _natural(f), s._P)
+    parts = []
+    for sp in s._spc:
+        f = sp.f if s._layout == 0 else sp.f.permute(0, 2, 1)  # -> [N_q2][N_q1][P]
+        parts.append(f)
+    f = torch.stack(parts, dim=2)  # [N_q2][N_q1][S][P]
+    write_vec(file_name + '.bin', _natural(f), s._P * s.N_species)
 
 
 def load_distribution_function(solver, file_name):
-    """load.py:10-43: f from file_name + '.bin' (ghost zones refilled
-    periodically, as globalToLocal does)."""
+    """load.py:10-43: f from file_name + '.bin' (ghost zones refilled by the
+    boundary conditions before the next RHS)."""
     s = solver
-    v = read_vec(file_name + '.bin', s._Q * s._P)
-    x = torch.from_numpy(v).to(s.device).reshape(s.N_q2, s.N_q1, s._P)
-    if s._layout != 0:
-        x = x.permute(0, 2, 1)
-    s._f = s._store(x.to(s._dtype))
-    s._update_moments()
+    S = s.N_species
+    v = read_vec(file_name + '.bin', s._Q * s._P * S)
+    x = torch.from_numpy(v).to(s.device).reshape(s.N_q2, s.N_q1, S, s._P)
+    for k in s._species_loop():
+        y = x[:, :, k]
+        if s._layout != 0:
+            y = y.permute(0, 2, 1)
+        s._f = s._store(y.to(s._dtype))
+        s._update_moments()
 
 
 def dump_moments(solver, file_name):

@@ -39,8 +39,44 @@ from .src.nonrelativistic_boltzmann import moments as _mom
 F64 = torch.float64
 
 
+class _Species(object):
+    """The state one species needs on the device: its grid (own p-range and
+    spacing, calculate_p.py:33-57), physics (own mass, charge, collision
+    time), f storage, moment block, workspace and work arrays.  Species are
+    coupled only through the fields (J and rho summed over species,
+    fields.py:562, 604-606)."""
+
+    def __init__(self):
+        self.grid = self.phys = self.f = self.mom = self.ws = None
+        self.bufs = {}
+        self.mass = self.charge = self.tau = 1.0
+        self.dp = [1.0, 1.0, 1.0]
+
+
+def _species_attr(name, doc):
+    def get(self):
+        return getattr(self._spc[self._cur], name)
+
+    def put(self, value):
+        setattr(self._spc[self._cur], name, value)
+    return property(get, put, doc=doc)
+
+
 class nonlinear_solver(object):
     """nonlinear_solver.py:55-521 for the FVM path on one GPU."""
+
+    # the current species' state (species 0 unless a loop over species is
+    # running); single-species code uses these names throughout
+    _grid = _species_attr('grid', 'boltx_grid of the current species')
+    _phys = _species_attr('phys', 'boltx_physics of the current species')
+    _f = _species_attr('f', 'f of the current species in the storage layout')
+    _mom = _species_attr('mom', 'FP64 moment block [12][Q] of the current species')
+    _ws = _species_attr('ws', 'workspace of the current species')
+    _bufs = _species_attr('bufs', 'work arrays of the current species')
+    _mass = _species_attr('mass', 'mass of the current species')
+    _charge = _species_attr('charge', 'charge of the current species')
+    _tau = _species_attr('tau', 'collision time of the current species')
+    _dp = _species_attr('dp', '[dp1, dp2, dp3] of the current species')
 
     def __init__(self, physical_system, performance_test_flag=False):
         self._L = _lib.load()
@@ -59,8 +95,6 @@ class nonlinear_solver(object):
         self.boundary_conditions = bc = ps.boundary_conditions
         self.N_species = ps.N_species
         self.performance_test_flag = performance_test_flag
-        if self.N_species != 1:
-            raise NotImplementedError('the fused kernels support one species')
         # nonlinear_solver.py:177-249 and boundaries.py:415-577
         sides = ('in_q1_left', 'in_q1_right', 'in_q2_bottom', 'in_q2_top')
         for side in sides:
@@ -86,8 +120,16 @@ class nonlinear_solver(object):
         # tensors (nonlinear_solver.py:133-145), the user's module untouched.
         self.params = types.SimpleNamespace(
             **{k: getattr(up, k) for k in dir(up) if not k.startswith('__')})
-        self._mass = float(up.mass[0])
-        self._charge = float(up.charge[0])
+        S = self.N_species
+        if len(up.mass) != S or len(up.charge) != S:
+            raise Exception('params.mass and params.charge need one entry per species')
+        # per-species state (grid, physics, storage, moments, workspace); the
+        # solver's single-species attributes (_grid, _phys, _f, _mom, ...)
+        # refer to species _cur
+        self._spc = [_Species() for _ in range(S)]
+        self._cur = 0
+        for k, sp in enumerate(self._spc):
+            sp.mass, sp.charge = float(up.mass[k]), float(up.charge[k])
         self.params.mass = torch.tensor([list(up.mass)], dtype=F64,
                                         device=self.device).reshape(1, -1, 1, 1)
         self.params.charge = torch.tensor([list(up.charge)], dtype=F64,
@@ -96,17 +138,18 @@ class nonlinear_solver(object):
 
         self._configure(ps, p)
         self._build_grids(ps)
-        self._grid = _lib.Grid(self.N_q1, self.N_q2, self.N_p1, self.N_p2, self.N_p3, 1,
-                               self._layout, float(self.dq1), float(self.dq2), self._dp[0], self._dp[1],
-                               self._dp[2], float(ps.p1_start[0]), float(ps.p2_start[0]),
-                               float(ps.p3_start[0]), int(self.N_ghost),
-                               (ctypes.c_int32 * 4)(*self._bc))
-        self._phys = _lib.Physics(self._recon_q, self._recon_p, self._riemann_q,
-                                  self._riemann_p, int(self._source_on), int(self._p_adv),
-                                  int(getattr(p, 'p_dim', 1)), self._precision,
-                                  self._mass, self._charge,
-                                  float(getattr(p, 'boltzmann_constant', 1.0)),
-                                  float(self._tau))
+        for k, sp in enumerate(self._spc):
+            sp.grid = _lib.Grid(self.N_q1, self.N_q2, self.N_p1, self.N_p2, self.N_p3, 1,
+                                self._layout, float(self.dq1), float(self.dq2), sp.dp[0],
+                                sp.dp[1], sp.dp[2], float(ps.p1_start[k]),
+                                float(ps.p2_start[k]), float(ps.p3_start[k]),
+                                int(self.N_ghost), (ctypes.c_int32 * 4)(*self._bc))
+            sp.phys = _lib.Physics(self._recon_q, self._recon_p, self._riemann_q,
+                                   self._riemann_p, int(self._source_on), int(self._p_adv),
+                                   int(getattr(p, 'p_dim', 1)), self._precision,
+                                   sp.mass, sp.charge,
+                                   float(getattr(p, 'boltzmann_constant', 1.0)),
+                                   float(sp.tau))
         Q, P = self.N_q1 * self.N_q2, self.N_p1 * self.N_p2 * self.N_p3
         self._Q, self._P = Q, P
         # ghost q-rows of the non-periodic axes after the Q interior rows
@@ -115,15 +158,16 @@ class nonlinear_solver(object):
         self._fshape = (self.N_q2, self.N_q1, P) if self._layout == 0 else \
             (self.N_q2, P, self.N_q1)
         self._dir_zero = None
-        self._bufs = {}
-        self._mom = torch.zeros(_lib.MOM_ROWS, Q, dtype=F64, device=self.device)
-        nws = int(self._L.boltx_workspace_size(ctypes.byref(self._grid), ctypes.byref(self._phys)))
-        self._ws = None
-        if nws > 0:
-            self._ws = torch.empty(nws, dtype=torch.uint8, device=self.device)
-            _lib.check(self._L.boltx_workspace_init(ctypes.byref(self._grid),
-                                                    ctypes.byref(self._phys),
-                                                    _lib.ptr(self._ws), _lib.stream_handle()))
+        for sp in self._spc:
+            sp.bufs = {}
+            sp.mom = torch.zeros(_lib.MOM_ROWS, Q, dtype=F64, device=self.device)
+            nws = int(self._L.boltx_workspace_size(ctypes.byref(sp.grid), ctypes.byref(sp.phys)))
+            sp.ws = None
+            if nws > 0:
+                sp.ws = torch.empty(nws, dtype=torch.uint8, device=self.device)
+                _lib.check(self._L.boltx_workspace_init(ctypes.byref(sp.grid),
+                                                        ctypes.byref(sp.phys),
+                                                        _lib.ptr(sp.ws), _lib.stream_handle()))
         self._flag = torch.zeros(1, dtype=torch.int32, device=self.device)
         self.time_elapsed = 0
         self.dt = None
@@ -214,7 +258,7 @@ class nonlinear_solver(object):
 
     def _build_grids(self, ps):
         """calculate_q.py:7-41 and calculate_p.py:8-57 as device tensors."""
-        dev, S = self.device, 1
+        dev = self.device
         i1 = torch.arange(-self.N_g1, self.N_q1 + self.N_g1, dtype=F64, device=dev)
         i2 = torch.arange(-self.N_g2, self.N_q2 + self.N_g2, dtype=F64, device=dev)
         Q1g, Q2g = len(i1), len(i2)
@@ -227,35 +271,48 @@ class nonlinear_solver(object):
         self.q2_left_center = self.q2_center_bot  # calculate_q.py:37
         self.q1_center = self.q1_left_bot + 0.5 * self.dq1
         self.q2_center = self.q2_left_bot + 0.5 * self.dq2
-        dp = [float(ps.dp1[0]), float(ps.dp2[0]), float(ps.dp3[0])]
-        starts = [float(ps.p1_start[0]), float(ps.p2_start[0]), float(ps.p3_start[0])]
+        S = self.N_species
         N = [self.N_p1, self.N_p2, self.N_p3]
-        self._dp = dp
+        meshes = {0.5: [], 0.0: []}
+        for k, sp in enumerate(self._spc):
+            sp.dp = [float(ps.dp1[k]), float(ps.dp2[k]), float(ps.dp3[k])]
+            starts = [float(ps.p1_start[k]), float(ps.p2_start[k]), float(ps.p3_start[k])]
+            for off in (0.5, 0.0):
+                ax = [starts[a] + (off + torch.arange(N[a], dtype=F64, device=dev)) * sp.dp[a]
+                      for a in range(3)]
+                g3, g2, g1 = torch.meshgrid(ax[2], ax[1], ax[0], indexing='ij')
+                meshes[off].append([g.reshape(-1) for g in (g1, g2, g3)])
 
-        def mesh(offset):
-            ax = [starts[k] + (offset + torch.arange(N[k], dtype=F64, device=dev)) * dp[k]
-                  for k in range(3)]
-            g3, g2, g1 = torch.meshgrid(ax[2], ax[1], ax[0], indexing='ij')
-            return [g.reshape(-1, S, 1, 1).contiguous() for g in (g1, g2, g3)]
+        def mesh(off):
+            # (P, S, 1, 1) per axis: species k in column k (calculate_p.py:33-57)
+            return [torch.stack([m[a] for m in meshes[off]], dim=1).reshape(-1, S, 1, 1)
+                    .contiguous() for a in range(3)]
 
         self.p1_center, self.p2_center, self.p3_center = mesh(0.5)
         self.p1_left, self.p2_bottom, self.p3_back = mesh(0.0)
         self.p2_left, self.p3_left = self.p2_center, self.p3_center
         self.p1_bottom, self.p3_bottom = self.p1_center, self.p3_center
         self.p1_back, self.p2_back = self.p1_center, self.p2_center
-        self.dp1 = torch.tensor(dp[0], dtype=F64, device=dev).reshape(1, S, 1, 1)
-        self.dp2 = torch.tensor(dp[1], dtype=F64, device=dev).reshape(1, S, 1, 1)
-        self.dp3 = torch.tensor(dp[2], dtype=F64, device=dev).reshape(1, S, 1, 1)
-        # collision time: params.tau(q1, q2, p1, p2, p3) must be uniform
-        self._tau = 1.0
+        self.dp1 = torch.tensor([sp.dp[0] for sp in self._spc], dtype=F64,
+                                device=dev).reshape(1, S, 1, 1)
+        self.dp2 = torch.tensor([sp.dp[1] for sp in self._spc], dtype=F64,
+                                device=dev).reshape(1, S, 1, 1)
+        self.dp3 = torch.tensor([sp.dp[2] for sp in self._spc], dtype=F64,
+                                device=dev).reshape(1, S, 1, 1)
+        # collision time: params.tau(q1, q2, p1, p2, p3) must be uniform per species
+        for sp in self._spc:
+            sp.tau = 1.0
         if self._source_on:
             tau = self.params.tau(self.q1_center, self.q2_center, self.p1_center,
                                   self.p2_center, self.p3_center)
             tau = torch.as_tensor(tau, dtype=F64, device=dev)
-            t0 = float(tau.reshape(-1)[0])
-            if not bool((tau == t0).all()):
-                raise NotImplementedError('the fused kernel needs a uniform tau')
-            self._tau = t0
+            tau = torch.broadcast_to(tau, (self.p1_center.shape[0], S) +
+                                     tuple(self.q1_center.shape[2:]))
+            for k, sp in enumerate(self._spc):
+                t0 = float(tau[:, k].reshape(-1)[0])
+                if not bool((tau[:, k] == t0).all()):
+                    raise NotImplementedError('the fused kernel needs a uniform tau per species')
+                sp.tau = t0
 
     # -- layout helpers -------------------------------------------------------
     def _q_interior(self, x):
@@ -390,33 +447,59 @@ class nonlinear_solver(object):
         _lib.check(self._L.boltx_fill_ghosts(ctypes.byref(self._grid), ctypes.byref(self._phys),
                                              _lib.ptr(u), dirs, _lib.stream_handle()))
 
+    def _species_loop(self):
+        """Iterate over the species, making each the current one; species 0
+        is current again afterwards."""
+        try:
+            for k in range(len(self._spc)):
+                self._cur = k
+                yield k
+        finally:
+            self._cur = 0
+
     @property
     def f(self):
         """The distribution function in the reference's (P, S, Q1g, Q2g)
         layout (a copy): periodic ghost zones filled from the interior,
         non-periodic ones as the boundary conditions fill them."""
-        self._fill_ghosts(self._f)
-        return self._to_reference(self._f)
+        parts = []
+        for _ in self._species_loop():
+            self._fill_ghosts(self._f)
+            parts.append(self._to_reference(self._f))
+        return parts[0] if len(parts) == 1 else torch.cat(parts, dim=1)
 
     @f.setter
     def f(self, value):
-        self._f = self._to_internal(value)
-        self._update_moments()
+        value = torch.as_tensor(value, device=self.device)
+        value = torch.broadcast_to(value, (self._P, self.N_species) +
+                                   tuple(self.q1_center.shape[2:]))
+        for k in self._species_loop():
+            self._f = self._to_internal(value[:, k:k + 1])
+            self._update_moments()
 
     # -- initialisation (nonlinear_solver.py:464-500) -------------------------
     def _initialize(self, params):
         f = self.physical_system.initial_conditions.initialize_f(
             self.q1_center, self.q2_center, self.p1_center, self.p2_center,
             self.p3_center, params)
-        self._f = self._to_internal(torch.as_tensor(f, dtype=F64, device=self.device))
+        self.f = torch.as_tensor(f, dtype=F64, device=self.device)
         self.f_initial = self._f.clone()
-        self._update_moments()
         self.fields_solver = _fields_solver(self) if self._fields_on else None
 
     def _update_moments(self):
+        """The moment block of the current species' f."""
         _lib.check(self._L.boltx_moments(ctypes.byref(self._grid), ctypes.byref(self._phys),
                                          _lib.ptr(self._f), _lib.ptr(self._mom),
                                          _lib.ptr(self._ws), _lib.stream_handle()))
+
+    def _rho(self, moms=None):
+        """Charge density summed over species (fields.py:562), [Q] FP64, from
+        the species' moment blocks (or ``moms``, one per species)."""
+        moms = moms or [sp.mom for sp in self._spc]
+        rho = self._spc[0].charge * moms[0][0]
+        for sp, m in zip(self._spc[1:], moms[1:]):
+            rho = rho + sp.charge * m[0]
+        return rho
 
     def _buf(self, name, dtype=None, shape=None):
         """A cached work array; f-sized ones (no shape given) have room for
@@ -432,22 +515,35 @@ class nonlinear_solver(object):
         return t
 
     # -- the step -------------------------------------------------------------
-    def _stage(self, u, w, un, out, a, b, cdt, mom_in, mom_out):
-        """One RK stage, out = a*un + b*(w + cdt*L(u)) (df_dt_fvm.py:33-339
-        inside fvm_operator.py:22-33)."""
+    def _fields_for_stage(self, us):
+        """The field work of one RHS evaluation (df_dt_fvm.py:102-104,
+        214-267): the electrostatic solve from rho, or the FDTD update from J
+        when at_n is False, with J and rho summed over species (fields.py:562,
+        604-606); then the three field arrays C_p1..C_p3 see, toggling at_n.
+        ``us`` are the species' stage inputs; their moment blocks are ``moms``
+        of the stage (set by the caller)."""
         F = [None, None, None]
-        self._fill_ghosts(u)  # fvm_operator.py:22-33: ghost refresh before each RHS
-        if self._p_adv:
-            fs = self.fields_solver
-            if self._es:  # df_dt_fvm.py:235-242: rho of the stage input
-                fs.compute_electrostatic_fields(self._charge * mom_in[0])
-            elif fs.at_n is False:  # df_dt_fvm.py:102-104, 214-233
-                J = self._buf('J', F64, (3, self._Q))
+        if not self._p_adv:
+            return F
+        fs = self.fields_solver
+        if self._es:  # df_dt_fvm.py:235-242: rho of the stage input
+            fs.compute_electrostatic_fields(self._rho(self._stage_moms))
+        elif fs.at_n is False:
+            J = None
+            for k in self._species_loop():
+                Jk = self._buf('J', F64, (3, self._Q))
                 _lib.check(self._L.boltx_currents(ctypes.byref(self._grid),
-                                                  ctypes.byref(self._phys), _lib.ptr(u),
-                                                  _lib.ptr(J), _lib.stream_handle()))
-                fs.evolve_electrodynamic_fields(J, self.dt)
-            F = [fs._take_for_C_p() for _ in range(3)]  # df_dt_fvm.py:251-267
+                                                  ctypes.byref(self._phys), _lib.ptr(us[k]),
+                                                  _lib.ptr(Jk), _lib.stream_handle()))
+                J = Jk if J is None else J + Jk
+            fs.evolve_electrodynamic_fields(J, self.dt)
+        return [fs._take_for_C_p() for _ in range(3)]  # df_dt_fvm.py:251-267
+
+    def _stage_kernel(self, u, w, un, out, a, b, cdt, mom_in, mom_out, F):
+        """The current species' stage launch, out = a*un + b*(w + cdt*L(u)),
+        with its ghost refresh (fvm_operator.py:22-33) and the moments of
+        out."""
+        self._fill_ghosts(u)
         # In the q1-line layout the moments of the output are a separate
         # streaming pass; issue it as its own call.
         split = self._layout == _lib.LAYOUT['q1-lines'] and mom_out is not None
@@ -470,24 +566,50 @@ class nonlinear_solver(object):
                                              _lib.ptr(out), _lib.ptr(mom_out), _lib.ptr(self._ws),
                                              _lib.stream_handle()))
 
+    def _stage(self, u, w, un, out, a, b, cdt, mom_in, mom_out):
+        """One RK stage of a one-species problem, out = a*un + b*(w + cdt*L(u))
+        (df_dt_fvm.py:33-339 inside fvm_operator.py:22-33)."""
+        self._stage_moms = [mom_in]
+        F = self._fields_for_stage([u])
+        self._stage_kernel(u, w, un, out, a, b, cdt, mom_in, mom_out, F)
+
     def _advance(self, dt):
         """Advance f by dt without the host-side divergence check (the device
-        flag is still set; see strang_timestep)."""
+        flag is still set; see strang_timestep).  Each stage evaluates the
+        fields once from every species, then runs every species' stage."""
         self.dt = dt
-        f = self._f
         if self._stepper == 'RK2':
             # fvm_operator.py:25-33
-            u1, m1 = self._buf('u1'), self._buf('m1', F64, (_lib.MOM_ROWS, self._Q))
-            self._stage(f, f, None, u1, 0.0, 1.0, dt / 2, self._mom, m1)
-            self._stage(u1, f, None, f, 0.0, 1.0, dt, m1, self._mom)
+            plan = [('f', 'f', None, 'u1', 0.0, 1.0, dt / 2, 'mom', 'm1'),
+                    ('u1', 'f', None, 'f', 0.0, 1.0, dt, 'm1', 'mom')]
         else:
-            u1, m1 = self._buf('u1'), self._buf('m1', F64, (_lib.MOM_ROWS, self._Q))
-            u2, m2 = self._buf('u2'), self._buf('m2', F64, (_lib.MOM_ROWS, self._Q))
-            self._stage(f, f, None, u1, 0.0, 1.0, dt, self._mom, m1)
-            self._stage(u1, u1, f, u2, 0.75, 0.25, dt, m1, m2)
-            self._stage(u2, u2, f, f, 1 / 3, 2 / 3, dt, m2, self._mom)
-        _lib.check(self._L.boltx_check_finite(ctypes.byref(self._grid), _lib.ptr(self._mom),
-                                              _lib.ptr(self._flag), _lib.stream_handle()))
+            plan = [('f', 'f', None, 'u1', 0.0, 1.0, dt, 'mom', 'm1'),
+                    ('u1', 'u1', 'f', 'u2', 0.75, 0.25, dt, 'm1', 'm2'),
+                    ('u2', 'u2', 'f', 'f', 1 / 3, 2 / 3, dt, 'm2', 'mom')]
+
+        def arr(name):
+            if name is None:
+                return None
+            if name == 'f':
+                return self._f
+            if name == 'mom':
+                return self._mom
+            if name in ('m1', 'm2'):
+                return self._buf(name, F64, (_lib.MOM_ROWS, self._Q))
+            return self._buf(name)
+
+        for (u, w, un, out, a, b, cdt, mi, mo) in plan:
+            args = []
+            for _ in self._species_loop():
+                args.append([arr(x) for x in (u, w, un, out, mi, mo)])
+            self._stage_moms = [x[4] for x in args]
+            F = self._fields_for_stage([x[0] for x in args])
+            for k in self._species_loop():
+                uu, ww, nn, oo, mmi, mmo = args[k]
+                self._stage_kernel(uu, ww, nn, oo, a, b, cdt, mmi, mmo, F)
+        for _ in self._species_loop():
+            _lib.check(self._L.boltx_check_finite(ctypes.byref(self._grid), _lib.ptr(self._mom),
+                                                  _lib.ptr(self._flag), _lib.stream_handle()))
         self.time_elapsed += dt
 
     def check_divergence(self):
@@ -520,7 +642,8 @@ class nonlinear_solver(object):
         evaluated with torch on the device."""
         if f is None and moment_name in _mom._BUILTIN and \
                 self.physical_system.moments is _mom:
-            return self._q_padded(self._mom[_mom._BUILTIN.index(moment_name)])
+            k = _mom._BUILTIN.index(moment_name)
+            return torch.cat([self._q_padded(sp.mom[k]) for sp in self._spc], dim=1)
         fn = getattr(self.physical_system.moments, moment_name)
         ff = self.f if f is None else torch.as_tensor(f, device=self.device)
         return fn(ff.to(F64), self.p1_center, self.p2_center, self.p3_center,
@@ -552,7 +675,7 @@ class nonlinear_solver(object):
             print('no timed steps (construct with performance_test_flag=True)')
             return
         t = self._timings['time_ts'] / n
-        cells = self._Q * self._P
+        cells = self._Q * self._P * self.N_species
         print('Time per step           : %.6e s' % t)
         print('Phase-space cell-updates/s: %.6e' % (cells / t))
         print('Spatial Zone Cycles/s   : %.6e' % (self._Q / t))

@@ -294,3 +294,111 @@ def case_bc(*, left="mirror", right="mirror", bottom="periodic", top="periodic",
     c.initialize.initialize_f = initialize_f
     c.dt, c.steps = dt, 4
     return c
+
+
+def case_species(*, S=2, masses=(1.0, 4.0), charges=(-1.0, 1.0), temps=(1.0, 0.5),
+                 drifts=(0.3, -0.2), p_max=(6.0, 3.0), taus=(0.05, 0.2), N_q1=16, N_q2=6,
+                 N_p1=8, N_p2=6, N_p3=1, p_dim=2, fields=False, fields_type="electrostatic",
+                 recon_q="weno5", recon_p="minmod", riemann="upwind-flux", stepper="RK2",
+                 source=True, precision="fp64", memory_layout="auto", amp=3e-2, B0=0.2,
+                 dt=1e-3):
+    """S species on their own p-grids ([-p_max_s, p_max_s] per active axis),
+    each a drifting Maxwellian with its own mass, temperature and collision
+    time, density 1 + amp cos(2 pi (q1 + q2)) (shifted in phase per species),
+    q periodic.  Fields optional: electrostatic (FFT) or electrodynamic
+    (FDTD, B = (0, 0, B0))."""
+    p_dim = min(p_dim, 3)
+    pm = list(p_max[:S])
+
+    def ext(N, k):
+        return ([-0.5] * S, [0.5] * S) if N == 1 else ([-v for v in pm], list(pm))
+    p1s, p1e = ext(N_p1, 0)
+    p2s, p2e = ext(N_p2, 1)
+    p3s, p3e = ext(N_p3, 2)
+    domain = _module("domain", q1_start=0.0, q1_end=1.0, N_q1=N_q1, q2_start=0.0,
+                     q2_end=1.0, N_q2=N_q2, p1_start=p1s, p1_end=p1e, N_p1=N_p1,
+                     p2_start=p2s, p2_end=p2e, N_p2=N_p2, p3_start=p3s, p3_end=p3e,
+                     N_p3=N_p3, N_ghost=3)
+    q2_bc = "periodic" if N_q2 > 1 else "none"
+    bc = _module("boundary_conditions", in_q1_left="periodic", in_q1_right="periodic",
+                 in_q2_bottom=q2_bc, in_q2_top=q2_bc)
+    tau_v = list(taus[:S])
+
+    def tau(q1, q2, p1, p2, p3):
+        t = torch.tensor(tau_v, dtype=torch.float64, device=p1.device).reshape(1, S, 1, 1)
+        return t * (q1 * 0 + 1) * (p1 * 0 + 1)
+
+    params = _module(
+        "params", fields_type=fields_type,
+        fields_initialize="fft" if fields_type == "electrostatic" else "user-defined",
+        fields_solver="fft" if fields_type == "electrostatic" else "fdtd",
+        solver_method_in_q="FVM", solver_method_in_p="FVM",
+        reconstruction_method_in_q=recon_q, reconstruction_method_in_p=recon_p,
+        riemann_solver_in_q=riemann, riemann_solver_in_p=riemann, p_dim=p_dim,
+        num_devices=1, mass=list(masses[:S]), boltzmann_constant=1, charge=list(charges[:S]),
+        fields_enabled=fields, source_enabled=source, instantaneous_collisions=False,
+        hybrid_model_enabled=False, eps=1, mu=1, tau=tau, time_stepper_fvm=stepper,
+        precision=precision, memory_layout=memory_layout, B0=B0, temps=list(temps[:S]),
+        drifts=list(drifts[:S]), amp=amp)
+
+    def initialize_f(q1, q2, p1, p2, p3, params):
+        m = params.mass
+        ph = torch.arange(S, dtype=torch.float64, device=p1.device).reshape(1, S, 1, 1)
+        T = torch.tensor(params.temps, dtype=torch.float64, device=p1.device).reshape(1, S, 1, 1)
+        u = torch.tensor(params.drifts, dtype=torch.float64, device=p1.device).reshape(1, S, 1, 1)
+        n = 1.0 + params.amp * torch.cos(2 * math.pi * (q1 + q2) + 0.7 * ph)
+        e = (p1 - u) ** 2
+        if params.p_dim >= 2:
+            e = e + p2 ** 2
+        if params.p_dim >= 3:
+            e = e + p3 ** 2
+        return n * (m / (2 * math.pi * T)) ** (params.p_dim / 2) * torch.exp(-m * e / (2 * T))
+
+    def initialize_E(q1, q2, params):
+        z = 0.0 * q1
+        return (z, z, z)
+
+    def initialize_B(q1, q2, params):
+        z = 0.0 * q1
+        return (z, z, z + params.B0)
+
+    init = _module("initialize", initialize_f=initialize_f, initialize_E=initialize_E,
+                   initialize_B=initialize_B)
+    c = types.SimpleNamespace(domain=domain, boundary_conditions=bc, params=params,
+                              initialize=init)
+    c.dt, c.steps = dt, 10
+    return c
+
+
+def single_species_of(case, k):
+    """Species k of a multi-species case as a one-species problem."""
+    def clone(m):
+        return _module(m.__name__, **{a: v for a, v in vars(m).items()
+                                      if not a.startswith("__")})
+    c = types.SimpleNamespace(domain=clone(case.domain), params=clone(case.params),
+                              boundary_conditions=case.boundary_conditions,
+                              initialize=clone(case.initialize), dt=case.dt, steps=case.steps)
+    d, p = c.domain, c.params
+    for a in ("p1_start", "p1_end", "p2_start", "p2_end", "p3_start", "p3_end"):
+        setattr(d, a, [getattr(d, a)[k]])
+    for a in ("mass", "charge", "temps", "drifts"):
+        setattr(p, a, [getattr(p, a)[k]])
+    tv = case.params.tau(torch.zeros(1, 1, 1, 1, dtype=torch.float64),
+                         torch.zeros(1, 1, 1, 1, dtype=torch.float64),
+                         torch.zeros(1, len(case.params.mass), 1, 1, dtype=torch.float64),
+                         None, None)[0, k].reshape(-1)[0].item()
+    p.tau = lambda q1, q2, p1, p2, p3, _t=tv: _t * (q1 * 0 + 1) * (p1 * 0 + 1)
+    f_all = case.initialize.initialize_f
+
+    def initialize_f(q1, q2, p1, p2, p3, params, _k=k):
+        # the multi-species initial condition with this species in column 0
+        S = len(case.params.mass)
+        pad = [torch.cat([x] * S, dim=1) if x is not None and x.shape[1] == 1 else x
+               for x in (p1, p2, p3)]
+        full_params = types.SimpleNamespace(**vars(case.params))
+        full_params.mass = torch.tensor(case.params.mass, dtype=torch.float64,
+                                        device=p1.device).reshape(1, S, 1, 1)
+        return f_all(q1, q2, *pad, full_params)[:, _k:_k + 1]
+
+    c.initialize.initialize_f = initialize_f
+    return c
