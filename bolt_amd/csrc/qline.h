@@ -144,9 +144,11 @@ __device__ inline void ql_put(T* line, int i, bool live, int n1, T v) {
 // QL_GENERAL   anything else, decided at run time.
 enum QlMode : int { QL_WU_PLAIN = 0, QL_WU_COMB = 1, QL_W_PLAIN = 2, QL_GENERAL = 3 };
 
-// Every f-sized stream (window rows of u, w, un, the output) is read or
-// written once per launch, so it goes through non-temporal accesses and the
-// L2 keeps the Maxwellian table rows the p-cells of a segment share.
+// Every f-sized input stream (window rows of u, w, un) is read once per
+// launch, so it goes through non-temporal loads and the L2 keeps the
+// Maxwellian table rows the p-cells of a segment share; the output is stored
+// normally, since the moment pass reads it right after (measured 1-2 % faster
+// per step than a non-temporal store).
 //
 // One wave's march over a q2 segment for the upwind directions P1 (C_q1 > 0)
 // and P2 (C_q2 > 0), fixed at compile time, in stage form MODE with PD
@@ -370,7 +372,7 @@ __device__ __forceinline__ void ql_march(const QLParams& q, int p, int seg, int 
       }
       const T tv = ql_fma(dk, tcdt, w_is_u ? f : WC[cu][k]);
       const T x = use_un ? ql_fma(ta, UNC[cu][k], tb * tv) : (plain ? tv : tb * tv);
-      if (live[k]) __builtin_nontemporal_store(x, orow + i1[k]);
+      if (live[k]) orow[i1[k]] = x;  // kept in the caches for the moment pass
     }
     rcur = rnx1;
     rnx1 = rnx1 + 1 == n2 ? 0 : rnx1 + 1;
