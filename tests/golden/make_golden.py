@@ -3,8 +3,9 @@
 The reference ships no golden vectors, so these are oracle outputs on small
 seeded problems: the 1D1V minmod/RK2 case after 1 and 100 steps, one step of a
 small 2D2V problem per reconstruction x stepper, one step of a small 2D3V
-electromagnetic problem and ten steps of a 1D1V electrostatic (FFT Poisson)
-problem.  They pin the oracle against accidental change and
+electromagnetic problem, ten steps of a 1D1V electrostatic (FFT Poisson)
+problem, five steps of a 2D2V box with mirror, dirichlet and mirror+dirichlet
+walls, and five steps of a two-species 2D2V problem.  They pin the oracle against accidental change and
 give the GPU tests fixed expected outputs.  Run from the repository root:
 
     python -m tests.golden.make_golden
@@ -18,7 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 def golden_cases():
     """name -> (case factory, steps)."""
-    from tests.cases import case_2d2v, case_2d3v, case_c1, case_es
+    from tests.cases import case_2d2v, case_2d3v, case_bc, case_c1, case_es, case_species
     out = {"c1_1step": (lambda: case_c1(seed=0), 1), "c1_100steps": (lambda: case_c1(seed=0), 100)}
     for rq in ("piecewise-constant", "minmod", "ppm", "weno5"):
         for st in ("RK2", "SSP-RK3"):
@@ -27,6 +28,10 @@ def golden_cases():
                                                    recon_p=rq, stepper=st), 1e-3), 1)
     out["2d3v_em"] = (lambda: _dt(case_2d3v(seed=2, N_q=6, N_p=4), 1e-3), 1)
     out["1d1v_electrostatic_10steps"] = (lambda: case_es(seed=3), 10)
+    out["2d2v_walls_5steps"] = (lambda: case_bc(
+        left="mirror", right="dirichlet", bottom="mirror+dirichlet", top="mirror",
+        recon_q="weno5", stepper="SSP-RK3", source=True), 5)
+    out["2d2v_two_species_5steps"] = (lambda: case_species(stepper="SSP-RK3"), 5)
     return out
 
 
