@@ -145,14 +145,26 @@ def run_em(steps=10, warmup=2):
         s = hip_solver(c)
         for _ in range(warmup):
             s._advance(c.dt)
+        events = []
+        s._stage_events = events
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
             s._advance(c.dt)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / steps
+        s._stage_events = None
         s.check_divergence()
-        out[prec] = {"ms_per_step": dt * 1e3, "value": s._Q * s._P / dt}
+        # the stage kernel (qcell_kernel with p-advection) against the HBM
+        # roofline at the algorithmic bytes of an SSP-RK3 stage (8/3 f-passes)
+        stage_s = sum(e0.elapsed_time(e1) for e0, e1 in events) / len(events) / 1e3
+        esz = 8 if prec == "fp64" else 4
+        achieved = 8 / 3 * s._Q * s._P * esz / stage_s / 1e9
+        out[prec] = {"ms_per_step": dt * 1e3, "value": s._Q * s._P / dt,
+                     "stage_launch_ms_avg": stage_s * 1e3,
+                     "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                                  "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                                  "traffic": None}}
         f[prec] = s._f.double()
         del s
     rel = (torch.linalg.vector_norm(f["fp32-bulk"] - f["fp64"]) /
