@@ -120,6 +120,18 @@ struct QCParams {
 template <bool PADV>
 constexpr int qc_block() { return PADV ? 1024 : 256; }
 
+// q-cell of workgroup b of nb = Q: the eight XCDs take contiguous bands of
+// q2 rows, and each walks its band q2-fastest, so a q-cell's stencil
+// neighbours (3 rows and 3 columns away) were loaded within the last few
+// dozen workgroups of that XCD and are still in its L2.
+__device__ inline int qc_order(const StageParams& sp, int b, int nb) {
+  if (nb % 8 != 0) return b;
+  const int l = b / 8, x = b % 8;
+  if (sp.n_q2 % 8 != 0 || sp.n_q2 < 8) return x * (nb / 8) + l;
+  const int rows = sp.n_q2 / 8;  // rows of the band
+  return (l / rows) + sp.n_q1 * (x * rows + l % rows);
+}
+
 template <typename T, int RQ, int RP, bool PADV>
 __global__ __launch_bounds__(qc_block<PADV>()) void qcell_kernel(QCParams qp) {
   constexpr int NW = qc_block<PADV>() / 64;
@@ -128,8 +140,7 @@ __global__ __launch_bounds__(qc_block<PADV>()) void qcell_kernel(QCParams qp) {
   const StageParams& sp = qp.sp;
   T* tab = reinterpret_cast<T*>(smem);
   T* blk = tab + (sp.n_p1 + sp.n_p2 + sp.n_p3);
-  const int nb = gridDim.x;
-  const int q = (nb % 8 == 0) ? (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : blockIdx.x;
+  const int q = qc_order(sp, blockIdx.x, gridDim.x);
   QCell<T> c;
   qcell_neighbours(sp, q, c);
   c.pref = 0.0;
