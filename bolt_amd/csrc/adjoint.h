@@ -118,16 +118,19 @@ __host__ __device__ inline long adj_index(const AdjParams& ap, int i1, int i2, i
 }
 
 // Per q-cell: reductions over p and the chain rule to the raw moments.
-__host__ __device__ inline void adj_reduce_cell(const AdjParams& ap, int q) {
+// The sums R0 = sum mu f_MB, R1k = sum mu f_MB (v_k - u_k), R2 = sum mu f_MB
+// |v - u|^2 and S = sum mu (f_MB - f) over p = p0, p0 + step, ... < P.
+__host__ __device__ inline void adj_reduce_sums(const AdjParams& ap, int q, int p0, int step,
+                                                double r[6]) {
   const int Q = ap.n1 * ap.n2;
   const int i1 = q % ap.n1, i2 = q / ap.n1;
   const double* b = ap.mom + MOM_BGK * Q + q;
-  const double A = b[0], u1 = b[Q], u2 = b[2 * Q], u3 = b[3 * Q], T = b[4 * Q], n = b[5 * Q];
+  const double A = b[0], u1 = b[Q], u2 = b[2 * Q], u3 = b[3 * Q];
   const double beta = b[6 * Q];
   const int d = ap.p_dim;
   double R0 = 0, R11 = 0, R12 = 0, R13 = 0, R2 = 0, S = 0;
-  int j1 = 0, j2 = 0, j3 = 0;
-  for (int p = 0; p < ap.P; ++p) {
+  int j1 = p0 % ap.n_p1, j2 = (p0 / ap.n_p1) % ap.n_p2, j3 = p0 / (ap.n_p1 * ap.n_p2);
+  for (int p = p0; p < ap.P; p += step) {
     const double v1 = centre(ap.p1s, j1, ap.dp1), v2 = centre(ap.p2s, j2, ap.dp2);
     const double v3 = centre(ap.p3s, j3, ap.dp3);
     const double d1 = v1 - u1, d2 = v2 - u2, d3 = v3 - u3;
@@ -144,14 +147,31 @@ __host__ __device__ inline void adj_reduce_cell(const AdjParams& ap, int q) {
     R13 += w * d3;
     R2 += w * q2;
     S += mu * (fmb - ap.u[idx]);
-    if (++j1 == ap.n_p1) {
-      j1 = 0;
+    j1 += step;
+    while (j1 >= ap.n_p1) {
+      j1 -= ap.n_p1;
       if (++j2 == ap.n_p2) {
         j2 = 0;
         ++j3;
       }
     }
   }
+  r[0] = R0;
+  r[1] = R11;
+  r[2] = R12;
+  r[3] = R13;
+  r[4] = R2;
+  r[5] = S;
+}
+
+// The chain rule from the sums of q-cell q to its moment weights (and the
+// tau gradient).
+__host__ __device__ inline void adj_reduce_finish(const AdjParams& ap, int q, const double r[6]) {
+  const int Q = ap.n1 * ap.n2;
+  const double* b = ap.mom + MOM_BGK * Q + q;
+  const double u1 = b[Q], u2 = b[2 * Q], u3 = b[3 * Q], T = b[4 * Q], n = b[5 * Q];
+  const int d = ap.p_dim;
+  const double R0 = r[0], R11 = r[1], R12 = r[2], R13 = r[3], R2 = r[4], S = r[5];
   const double eps = 1e-30, m = ap.mass, kB = ap.kB, ne = n + eps;
   // derivatives of G = sum_p mu f_MB with respect to n, u_k, T
   const double Gn = R0 / n;
@@ -176,29 +196,65 @@ __host__ __device__ inline void adj_reduce_cell(const AdjParams& ap, int q) {
   if (ap.tau_grad) ap.tau_grad[q] = -ap.c * ap.inv_tau * ap.inv_tau * S;
 }
 
+// The same for one q-cell, summed in plain p order.
+__host__ __device__ inline void adj_reduce_cell(const AdjParams& ap, int q) {
+  double r[6];
+  adj_reduce_sums(ap, q, 0, 1, r);
+  adj_reduce_finish(ap, q, r);
+}
+
 #ifndef BX_NO_KERNELS  // host-only builds of the per-cell code leave the kernels out
-__global__ void adj_bgk_reduce_kernel(AdjParams ap) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < ap.n1 * ap.n2) adj_reduce_cell(ap, q);
+constexpr int ADJ_RW = 8;  // waves of a reduction workgroup
+
+// q1-line layout: a workgroup takes 64 consecutive q1 cells of one q2 row
+// (lane = q1 cell, coalesced along the q1 lines) and its ADJ_RW waves take
+// interleaved p; the waves' sums are added in wave order.
+__global__ __launch_bounds__(64 * ADJ_RW) void adj_bgk_reduce_lines_kernel(AdjParams ap) {
+  __shared__ double red[6][ADJ_RW][64];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int nblk = (ap.n1 + 63) / 64;
+  const int i1 = (blockIdx.x % nblk) * 64 + lane, i2 = blockIdx.x / nblk;
+  const bool live = i1 < ap.n1;
+  const int q = i2 * ap.n1 + (live ? i1 : 0);
+  double r[6] = {0, 0, 0, 0, 0, 0};
+  if (live) adj_reduce_sums(ap, q, wave, ADJ_RW, r);
+  for (int k = 0; k < 6; ++k) red[k][wave][lane] = r[k];
+  __syncthreads();
+  if (wave == 0 && live) {
+    for (int k = 0; k < 6; ++k) {
+      double x = red[k][0][lane];
+      for (int w = 1; w < ADJ_RW; ++w) x += red[k][w][lane];
+      r[k] = x;
+    }
+    adj_reduce_finish(ap, q, r);
+  }
+}
+
+// Reference layout: a workgroup per q-cell, threads over p (contiguous),
+// a butterfly within each wave and the waves added in order.
+__global__ __launch_bounds__(256) void adj_bgk_reduce_kernel(AdjParams ap) {
+  __shared__ double red[6][4];
+  const int q = blockIdx.x;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double r[6];
+  adj_reduce_sums(ap, q, threadIdx.x, 256, r);
+  for (int k = 0; k < 6; ++k) {
+    double x = r[k];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    if (lane == 0) red[k][wave] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < 6; ++k) r[k] = ((red[k][0] + red[k][1]) + red[k][2]) + red[k][3];
+    adj_reduce_finish(ap, q, r);
+  }
 }
 #endif
 
-// Cell t (memory order): out = a*base + b*mu + c*(J^T mu).
+// Cell (i1, i2, p) at memory offset t: out = a*base + b*mu + c*(J^T mu).
 template <int R>
-__host__ __device__ inline void adj_stage_cell(const AdjParams& ap, long t) {
+__host__ __device__ inline void adj_stage_at(const AdjParams& ap, int i1, int i2, int p, long t) {
   const long Qn = (long)ap.n1 * ap.n2;
-  int i1, i2, p;
-  if (ap.layout) {
-    i1 = (int)(t % ap.n1);
-    const long r = t / ap.n1;
-    p = (int)(r % ap.P);
-    i2 = (int)(r / ap.P);
-  } else {
-    p = (int)(t % ap.P);
-    const long r = t / ap.P;
-    i1 = (int)(r % ap.n1);
-    i2 = (int)(r / ap.n1);
-  }
   const int j1 = p % ap.n_p1, j2 = (p / ap.n_p1) % ap.n_p2, j3 = p / (ap.n_p1 * ap.n_p2);
   const double v1 = centre(ap.p1s, j1, ap.dp1), v2 = centre(ap.p2s, j2, ap.dp2);
   const double v3 = centre(ap.p3s, j3, ap.dp3);
@@ -231,11 +287,26 @@ __host__ __device__ inline void adj_stage_cell(const AdjParams& ap, long t) {
   ap.out[t] = x;
 }
 
+// The same for cell t of memory order (either layout keeps i2 outermost).
+template <int R>
+__host__ __device__ inline void adj_stage_cell(const AdjParams& ap, long t) {
+  const int slab = ap.n1 * ap.P;
+  const int i2 = (int)(t / slab), x = (int)(t % slab);
+  if (ap.layout) adj_stage_at<R>(ap, x % ap.n1, i2, x / ap.n1, t);
+  else adj_stage_at<R>(ap, x / ap.P, i2, x % ap.P, t);
+}
+
 #ifndef BX_NO_KERNELS
+// Grid: x over the n1*P cells of one q2 row (< 2^31, checked by the host), y = i2.
 template <int R>
 __global__ __launch_bounds__(256) void adj_stage_kernel(AdjParams ap) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < (long)ap.n1 * ap.n2 * ap.P) adj_stage_cell<R>(ap, t);
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int slab = ap.n1 * ap.P;
+  if (x >= slab) return;
+  const int i2 = blockIdx.y;
+  const long t = (long)i2 * slab + x;
+  if (ap.layout) adj_stage_at<R>(ap, x % ap.n1, i2, x / ap.n1, t);
+  else adj_stage_at<R>(ap, x / ap.P, i2, x % ap.P, t);
 }
 #endif
 

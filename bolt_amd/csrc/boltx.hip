@@ -590,8 +590,8 @@ static int ql_stage(const boltx_grid* g, const boltx_physics* ph, const boltx_st
   double* part = (double*)(ws + ql_table_bytes(g, ph->precision));
   int rc;
   if (sp.source) {
-    const long n = (long)(g->n_p1 + g->n_p2 + g->n_p3) * sp.Q;
-    hipLaunchKernelGGL(bgk_table_kernel<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+    const dim3 tg((unsigned)((sp.Q + 255) / 256), (unsigned)(g->n_p1 + g->n_p2 + g->n_p3));
+    hipLaunchKernelGGL(bgk_table_kernel<T>, tg, dim3(256), 0, s,
                        st->mom_in, sp.Q, g->n_p1, g->n_p2, g->n_p3, g->p1_start, g->dp1,
                        g->p2_start, g->dp2, g->p3_start, g->dp3, sp.mass, sp.kB, tab);
     if ((rc = launch_status("bgk_table_kernel"))) return rc;
@@ -896,26 +896,30 @@ extern "C" int boltx_adjoint_stage(const boltx_grid* grid, const boltx_physics* 
   const int need = recon_reach(phys->recon_q) + 1;
   if ((grid->n_q1 > 1 && grid->n_q1 < need) || (grid->n_q2 > 1 && grid->n_q2 < need))
     return fail(BOLTX_EINVAL, "q-grid smaller than the stencil reach");
-  if (ghost_rows(grid) > 0 && grid->n_ghost < need)
-    return fail(BOLTX_EINVAL, "n_ghost = %d is below the stencil reach %d", grid->n_ghost, need);
-  if (ghost_rows(grid) > 0 && phys->p_advection)
-    return fail(BOLTX_EUNSUPPORTED, "non-periodic q boundaries with p-advection");
+  if (ghost_rows(grid) > 0)
+    return fail(BOLTX_EUNSUPPORTED, "the adjoint needs periodic q boundaries");
   const AdjParams ap = adj_params(grid, phys, adj);
   hipStream_t s = (hipStream_t)stream;
   const long Q = (long)ap.n1 * ap.n2;
   if (ap.source) {
-    hipLaunchKernelGGL(adj_bgk_reduce_kernel, dim3((unsigned)((Q + 255) / 256)), dim3(256), 0, s,
-                       ap);
+    if (ap.layout) {
+      const unsigned nb1 = (unsigned)((ap.n1 + 63) / 64) * ap.n2;
+      hipLaunchKernelGGL(adj_bgk_reduce_lines_kernel, dim3(nb1), dim3(64 * ADJ_RW), 0, s, ap);
+    } else {
+      hipLaunchKernelGGL(adj_bgk_reduce_kernel, dim3((unsigned)Q), dim3(256), 0, s, ap);
+    }
     if ((rc = launch_status("adj_bgk_reduce_kernel"))) return rc;
   }
-  const unsigned nb = (unsigned)((Q * ap.P + 255) / 256);
+  if ((long)ap.n1 * ap.P >= (1L << 31) || ap.n2 > 65535)
+    return fail(BOLTX_EUNSUPPORTED, "the adjoint grid takes n_q1*P < 2^31 and n_q2 <= 65535");
+  const dim3 nb((unsigned)(((long)ap.n1 * ap.P + 255) / 256), (unsigned)ap.n2);
   switch (phys->recon_q) {
-    case RC_PC: hipLaunchKernelGGL(adj_stage_kernel<RC_PC>, dim3(nb), dim3(256), 0, s, ap); break;
+    case RC_PC: hipLaunchKernelGGL(adj_stage_kernel<RC_PC>, nb, dim3(256), 0, s, ap); break;
     case RC_MINMOD:
-      hipLaunchKernelGGL(adj_stage_kernel<RC_MINMOD>, dim3(nb), dim3(256), 0, s, ap);
+      hipLaunchKernelGGL(adj_stage_kernel<RC_MINMOD>, nb, dim3(256), 0, s, ap);
       break;
-    case RC_PPM: hipLaunchKernelGGL(adj_stage_kernel<RC_PPM>, dim3(nb), dim3(256), 0, s, ap); break;
-    default: hipLaunchKernelGGL(adj_stage_kernel<RC_WENO5>, dim3(nb), dim3(256), 0, s, ap);
+    case RC_PPM: hipLaunchKernelGGL(adj_stage_kernel<RC_PPM>, nb, dim3(256), 0, s, ap); break;
+    default: hipLaunchKernelGGL(adj_stage_kernel<RC_WENO5>, nb, dim3(256), 0, s, ap);
   }
   return launch_status("adj_stage_kernel");
 }

@@ -178,14 +178,14 @@ __global__ void ml_finalize_kernel(const double* part, int nchunk, int Q, double
 // Maxwellian tables of every q-cell from its moment block
 // (collision_operator.py:52-65): tab[j1][q] = prefactor * exp(-m (v1-u1)^2/2kT),
 // tab[n_p1 + j2][q] = exp(-m (v2-u2)^2/2kT), tab[n_p1 + n_p2 + j3][q] likewise.
+// Grid: x over q-cells, y = table row j (no 64-bit index arithmetic).
 template <typename T>
 __global__ void bgk_table_kernel(const double* mom, int Q, int n_p1, int n_p2, int n_p3,
                                  double p1s, double dp1, double p2s, double dp2, double p3s,
                                  double dp3, double mass, double kB, T* tab) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int nj = n_p1 + n_p2 + n_p3;
-  if (t >= (long)nj * Q) return;
-  const int j = (int)(t / Q), qc = (int)(t % Q);
+  const int qc = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;  // uniform over the workgroup
+  if (qc >= Q) return;
   const double* b = mom + MOM_BGK * Q + qc;
   const double Tq = b[4 * Q];
   double val;
@@ -193,7 +193,7 @@ __global__ void bgk_table_kernel(const double* mom, int Q, int n_p1, int n_p2, i
   else if (j < n_p1 + n_p2)
     val = maxwell_factor(centre(p2s, j - n_p1, dp2), b[2 * Q], Tq, mass, kB);
   else val = maxwell_factor(centre(p3s, j - n_p1 - n_p2, dp3), b[3 * Q], Tq, mass, kB);
-  tab[t] = T(val);
+  tab[(long)j * Q + qc] = T(val);
 }
 
 }  // namespace bx
