@@ -482,7 +482,7 @@ static int launch_qc_t(const QCParams& qp, hipStream_t s) {
   size_t lds = sizeof(T) * (size_t)(sp.n_p1 + sp.n_p2 + sp.n_p3);
   if (PADV) lds += sizeof(T) * (size_t)sp.P;
   if (lds > 160 * 1024) return fail(BOLTX_EUNSUPPORTED, "p-block of %zu bytes exceeds LDS", lds);
-  hipLaunchKernelGGL((qcell_kernel<T, RQ, RP, PADV>), dim3(sp.Q), dim3(qc_block<PADV>()), lds, s,
+  hipLaunchKernelGGL((qcell_kernel<T, RQ, RP, PADV>), dim3(sp.Q), dim3(qc_block<T, PADV>()), lds, s,
                      qp);
   return launch_status("qcell_kernel");
 }

@@ -141,6 +141,27 @@ def test_2d3v_fields_run(gpu, stepper):
     assert np.abs(a - b).max() <= 1e-11 * max(np.abs(b).max(), 1e-30)
 
 
+def test_2d3v_fields_fp32_bulk_vs_fp64_oracle(gpu):
+    """The FP32-bulk electromagnetic stage (p-block in LDS, two workgroups
+    per CU) against the FP64 oracle: f and the cell-centred fields."""
+    c64 = case_2d3v(seed=6, N_q=8, N_p=10)
+    c32 = case_2d3v(seed=6, N_q=8, N_p=10, precision="fp32-bulk")
+    o, nls = oracle_solver(c64), hip_solver(c32)
+    for _ in range(6):
+        o.strang_timestep(c64.dt)
+        nls.strang_timestep(c32.dt)
+    linf, l2 = _cmp(nls, o)
+    print(f"2D3V FP32-bulk 6 steps vs FP64 oracle: rel L2 {l2:.3e} rel Linf {linf:.3e}",
+          flush=True)
+    assert l2 <= 1e-5
+    a = nls.fields_solver.cell_centered_EM_fields.cpu().numpy()
+    b = o.fields_solver.cell_centered_EM_fields.numpy()
+    g1, g2 = nls.N_g1, nls.N_g2
+    a = a[:, :, g1:g1 + nls.N_q1, g2:g2 + nls.N_q2]
+    b = b[:, :, g1:g1 + nls.N_q1, g2:g2 + nls.N_q2]
+    assert np.abs(a - b).max() <= 1e-5 * max(np.abs(b).max(), 1e-30)
+
+
 def test_divergence_raises(gpu):
     c = case_c1(seed=0)
     nls = hip_solver(c)
