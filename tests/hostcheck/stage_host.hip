@@ -143,8 +143,20 @@ extern "C" int host_adjoint_stage(const boltx_grid* g, const boltx_physics* ph,
                                   const boltx_adjoint* adj) {
   const AdjParams ap = adj_params(g, ph, adj);
   const int Q = ap.n1 * ap.n2;
+  // the reduction over p split as the device kernels split it: interleaved
+  // p-ranges (8 in the q1-line layout, 256 in the reference layout) whose
+  // sums are added in order
+  const int split = ap.layout ? 8 : 256;
   if (ap.source)
-    for (int q = 0; q < Q; ++q) adj_reduce_cell(ap, q);
+    for (int q = 0; q < Q; ++q) {
+      double r[6] = {0, 0, 0, 0, 0, 0};
+      for (int w = 0; w < split; ++w) {
+        double x[6];
+        adj_reduce_sums(ap, q, w, split, x);
+        for (int k = 0; k < 6; ++k) r[k] += x[k];
+      }
+      adj_reduce_finish(ap, q, r);
+    }
   const long n = (long)Q * ap.P;
   for (long t = 0; t < n; ++t) {
     switch (ph->recon_q) {

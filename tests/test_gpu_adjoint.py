@@ -53,6 +53,38 @@ def test_stage_adjoint_vs_oracle_autograd(gpu, layout, rq):
     assert err <= 1e-10 and terr <= 1e-10
 
 
+@pytest.mark.parametrize("layout,n1,np1,np2", [("q1-lines", 40, 12, 10),
+                                               ("reference", 10, 20, 15)])
+def test_stage_adjoint_ragged_reduction(gpu, layout, n1, np1, np2):
+    """The p-parallel BGK reduction of the adjoint with q1 lines that leave
+    lanes of a 64-wide block empty (n1 = 40) and with more p-cells than the
+    256 threads of a q-cell's workgroup (P = 300)."""
+    from bolt_amd import adjoint as A
+    c = maxwellian_case(N_q1=n1, N_q2=7, N_p1=np1, N_p2=np2, p_dim=2, recon_q="weno5",
+                        recon_p="weno5", riemann="lax-friedrichs", tau=0.04, memory_layout=layout,
+                        modes=((1, 1),), amps=(3e-2,), phases=(0.3,))
+    o = oracle_solver(c)
+    gen = torch.Generator().manual_seed(12)
+    u = o.interior().clone() * (1 + 0.2 * torch.rand(o.interior().shape, generator=gen, dtype=F64))
+    mu = torch.randn(u.shape, generator=gen, dtype=F64)
+    gu, gt = _oracle_vjp(o, u, mu, 0.04)
+    s = hip_solver(c)
+    U = A.interior_to_internal(s, u[:, 0].to(gpu))
+    MU = A.interior_to_internal(s, mu[:, 0].to(gpu))
+    mom = A._moments_of(s, U)
+    ws = torch.empty(5 * s._Q * 8, dtype=torch.uint8, device=gpu)
+    tq = torch.empty(s._Q, dtype=F64, device=gpu)
+    out = A._adjoint_stage(s, U, mom, MU, None, 0.0, 0.0, 1.0, ws, tq)
+    got = A.internal_to_interior(s, out).cpu()
+    err = float((got - gu).abs().max() / gu.abs().max())
+    terr = abs(float(tq.sum()) - gt) / abs(gt)
+    print(f"{layout} n1={n1} P={np1 * np2}: J^T mu rel Linf {err:.3e}, dL/dtau rel {terr:.3e}",
+          flush=True)
+    assert err <= 1e-10 and terr <= 1e-10
+    out2 = A._adjoint_stage(s, U, mom, MU, None, 0.0, 0.0, 1.0, ws, tq)
+    assert torch.equal(out, out2)  # fixed summation order: bit-reproducible
+
+
 def _c4_loss(n, E, dq):
     return ((n - 1) ** 2).sum() * dq + E.sum() * dq
 
