@@ -60,30 +60,56 @@ struct EnsStageArgs {
 // n1 >= 5) or holds one line of up to 256 cells.
 constexpr int ENS_LDS = 264;
 
+// Everything the stage reads from memory for one line of a wave (the lane's
+// cells of it): u, the Maxwellian table factors, w and un, and the line's C1.
+// A wave fetches the next line into one of two such slots while it works on
+// the other, and the slots swap roles from line to line, so no loaded value
+// is copied (a copy would wait for the loads still in flight).
+template <typename T, int NR>
+struct EnsLineIn {
+  T f[NR], t1[NR], t2[NR], t3[NR], wc[NR], unc[NR];
+  T C1;
+};
+
+// Fetch line p (velocity indices j1, j2, j3) for lane cells
+// lane_in_line + 64 k; nothing is loaded for dead lanes or a dead line.
+template <typename T, int NR>
+__device__ inline void ens_fetch(const EnsStageArgs& ea, const EnsInst& I, int p, int j1, int j2,
+                                 int j3, int lane_in_line, bool live_line, EnsLineIn<T, NR>& in) {
+  const int n1 = I.n1;
+  const long base = I.off_f + (long)p * n1;
+  const T* u = (const T*)ea.u + base;
+  const T* tab = (const T*)ea.tab + I.off_tab;
+  const T* wv = (const T*)ea.w + base;
+  const T* unv = (const T*)ea.un + base;
+  const bool w_is_u = ea.w == ea.u;
+  const bool src = I.source != 0, src2 = src && I.p_dim >= 2, src3 = src && I.p_dim >= 3;
+  in.C1 = T(centre(I.p1s, j1, I.dp1));
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    const int i = lane_in_line + 64 * k;
+    const bool live = live_line && i < n1;
+    in.f[k] = live ? __builtin_nontemporal_load(u + i) : T(0);
+    in.t1[k] = live && src ? tab[(long)j1 * n1 + i] : T(0);
+    in.t2[k] = live && src2 ? tab[(long)(I.n_p1 + j2) * n1 + i] : T(0);
+    in.t3[k] = live && src3 ? tab[(long)(I.n_p1 + I.n_p2 + j3) * n1 + i] : T(0);
+    in.wc[k] = live && !w_is_u ? __builtin_nontemporal_load(wv + i) : T(0);
+    in.unc[k] = live && ea.use_un ? __builtin_nontemporal_load(unv + i) : T(0);
+  }
+}
+
 // Stencil, source and RK combination of the q1 lines of one wave.  Lane l
 // works on cells i1 = c0 + 64 k (k < NR) of the line starting at ``line``
 // (LDS, halo-padded; p-cell p), all lanes of a line sharing C1 and the table
 // rows.
-// The values of u on a lane's cells of the line of p-cell p (0 where dead).
-template <typename T, int NR>
-__device__ inline void ens_load_line(const EnsStageArgs& ea, const EnsInst& I, int p,
-                                     int lane_in_line, bool live_line, T f[NR]) {
-  const T* u = (const T*)ea.u + I.off_f + (long)p * I.n1;
-#pragma unroll
-  for (int k = 0; k < NR; ++k) {
-    const int i = lane_in_line + 64 * k;
-    f[k] = live_line && i < I.n1 ? __builtin_nontemporal_load(u + i) : T(0);
-  }
-}
-
 template <typename T, int R, int NR>
-__device__ inline void ens_lines(const EnsStageArgs& ea, const EnsInst& I, int p, int j1, int j2,
-                                 int j3, int lane_in_line, bool live_line, const T f[NR], T* fl,
-                                 T* kl, T* sl) {
+__device__ inline void ens_lines(const EnsStageArgs& ea, const EnsInst& I, int p, int lane_in_line,
+                                 bool live_line, const EnsLineIn<T, NR>& in, T* fl, T* kl, T* sl) {
   const int n1 = I.n1;
-  const T C1 = T(centre(I.p1s, j1, I.dp1));
+  const T C1 = in.C1;
   const bool pos = C1 > T(0);
   const long base = I.off_f + (long)p * n1;
+  const bool w_is_u = ea.w == ea.u;
   int i1[NR];
   bool live[NR];
 #pragma unroll
@@ -92,28 +118,8 @@ __device__ inline void ens_lines(const EnsStageArgs& ea, const EnsInst& I, int p
     live[k] = live_line && i1[k] < n1;
     if (!live[k]) i1[k] = 0;
   }
-  // pointwise inputs, issued before the LDS work so it covers their latency
-  const T* tab = (const T*)ea.tab + I.off_tab;
-  const T* wv = (const T*)ea.w + base;
-  const T* unv = (const T*)ea.un + base;
-  const bool w_is_u = ea.w == ea.u;
-  T fmb[NR], wc[NR], unc[NR];
 #pragma unroll
-  for (int k = 0; k < NR; ++k) {
-    const int i = i1[k];
-    fmb[k] = wc[k] = unc[k] = T(0);
-    if (live[k]) {
-      if (I.source) {
-        fmb[k] = tab[(long)j1 * n1 + i];
-        if (I.p_dim >= 2) fmb[k] = fmb[k] * tab[(long)(I.n_p1 + j2) * n1 + i];
-        if (I.p_dim >= 3) fmb[k] = fmb[k] * tab[(long)(I.n_p1 + I.n_p2 + j3) * n1 + i];
-      }
-      if (!w_is_u) wc[k] = __builtin_nontemporal_load(wv + i);
-      if (ea.use_un) unc[k] = __builtin_nontemporal_load(unv + i);
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < NR; ++k) ql_put(fl, i1[k], live[k], n1, f[k]);
+  for (int k = 0; k < NR; ++k) ql_put(fl, i1[k], live[k], n1, in.f[k]);
   __builtin_amdgcn_wave_barrier();
   T m3[NR], m2[NR], m1[NR], p1[NR], p2[NR], kap[NR];
 #pragma unroll
@@ -124,7 +130,7 @@ __device__ inline void ens_lines(const EnsStageArgs& ea, const EnsInst& I, int p
     m1[k] = line[-1];
     p1[k] = line[1];
     p2[k] = line[2];
-    kap[k] = ql_kappa<R, T>(m1[k], f[k], p1[k]);
+    kap[k] = ql_kappa<R, T>(m1[k], in.f[k], p1[k]);
     ql_put(kl, i1[k], live[k], n1, kap[k]);
   }
   __builtin_amdgcn_wave_barrier();
@@ -138,8 +144,8 @@ __device__ inline void ens_lines(const EnsStageArgs& ea, const EnsInst& I, int p
       kp1 = kl[i1[k] + 1];
     }
     // lower face of cell i: right face of i-1 (C > 0) or left face of i
-    s[k] = pos ? ql_face<R, true, T>(m3[k], m2[k], m1[k], f[k], p1[k], km2, km1, kap[k])
-               : ql_face<R, false, T>(m2[k], m1[k], f[k], p1[k], p2[k], km1, kap[k], kp1);
+    s[k] = pos ? ql_face<R, true, T>(m3[k], m2[k], m1[k], in.f[k], p1[k], km2, km1, kap[k])
+               : ql_face<R, false, T>(m2[k], m1[k], in.f[k], p1[k], p2[k], km1, kap[k], kp1);
     ql_put(sl, i1[k], live[k], n1, s[k]);
   }
   __builtin_amdgcn_wave_barrier();
@@ -150,38 +156,56 @@ __device__ inline void ens_lines(const EnsStageArgs& ea, const EnsInst& I, int p
   for (int k = 0; k < NR; ++k) {
     if (!live[k]) continue;
     const int i = i1[k];
+    const T f = in.f[k];
     const T su = sl[i + 1];
     T d = -(C1 * su - C1 * s[k]) * tinv;
-    if (I.source) d = d + (fmb[k] - f[k]) * ttau;
-    const T tv = (w_is_u ? f[k] : wc[k]) + d * tcdt;
-    __builtin_nontemporal_store(ea.use_un ? ta * unc[k] + tb * tv : (ea.b_one ? tv : tb * tv),
+    if (I.source) {
+      T fmb = in.t1[k];
+      if (I.p_dim >= 2) fmb = fmb * in.t2[k];
+      if (I.p_dim >= 3) fmb = fmb * in.t3[k];
+      d = d + (fmb - f) * ttau;
+    }
+    const T tv = (w_is_u ? f : in.wc[k]) + d * tcdt;
+    __builtin_nontemporal_store(ea.use_un ? ta * in.unc[k] + tb * tv : (ea.b_one ? tv : tb * tv),
                                 out + i);
   }
   __builtin_amdgcn_wave_barrier();
 }
 
+// (j1, j2, j3) of the p-cell ``step`` further on.
+__device__ inline void ens_advance(const EnsInst& I, int step, int& j1, int& j2, int& j3) {
+  j1 += step;
+  while (j1 >= I.n_p1) {
+    j1 -= I.n_p1;
+    if (++j2 == I.n_p2) {
+      j2 = 0;
+      ++j3;
+    }
+  }
+}
+
 // A wave task of lines longer than 64 cells: one line at a time, the next
-// line's values loaded while the current one is worked on.
+// line's inputs fetched while the current one is worked on.
 template <typename T, int R, int NR>
 __device__ inline void ens_task(const EnsStageArgs& ea, const EnsInst& I, const EnsTile& tile,
                                 int lane, T* fl, T* kl, T* sl) {
+  // indices of the next line to fetch
   int j1 = tile.p0 % I.n_p1, j23 = tile.p0 / I.n_p1;
   int j2 = j23 % I.n_p2, j3 = j23 / I.n_p2;
-  T cur[NR], nxt[NR];
-  ens_load_line<T, NR>(ea, I, tile.p0, lane, true, cur);
-  for (int q = 0; q < tile.np; ++q) {
-    const bool more = q + 1 < tile.np;
-    ens_load_line<T, NR>(ea, I, more ? tile.p0 + q + 1 : tile.p0, lane, more, nxt);
-    ens_lines<T, R, NR>(ea, I, tile.p0 + q, j1, j2, j3, lane, true, cur, fl, kl, sl);
-#pragma unroll
-    for (int k = 0; k < NR; ++k) cur[k] = nxt[k];
-    if (++j1 == I.n_p1) {
-      j1 = 0;
-      if (++j2 == I.n_p2) {
-        j2 = 0;
-        ++j3;
-      }
-    }
+  EnsLineIn<T, NR> A, B;
+  ens_fetch<T, NR>(ea, I, tile.p0, j1, j2, j3, lane, true, A);
+  ens_advance(I, 1, j1, j2, j3);
+  for (int q = 0; q < tile.np;) {
+    bool more = q + 1 < tile.np;
+    ens_fetch<T, NR>(ea, I, more ? tile.p0 + q + 1 : tile.p0, j1, j2, j3, lane, more, B);
+    ens_advance(I, 1, j1, j2, j3);
+    ens_lines<T, R, NR>(ea, I, tile.p0 + q, lane, true, A, fl, kl, sl);
+    if (++q == tile.np) break;
+    more = q + 1 < tile.np;
+    ens_fetch<T, NR>(ea, I, more ? tile.p0 + q + 1 : tile.p0, j1, j2, j3, lane, more, A);
+    ens_advance(I, 1, j1, j2, j3);
+    ens_lines<T, R, NR>(ea, I, tile.p0 + q, lane, true, B, fl, kl, sl);
+    ++q;
   }
 }
 
@@ -206,27 +230,32 @@ __global__ __launch_bounds__(256) void ens_stage_kernel(EnsStageArgs ea) {
       const int line = lane / I.n1, c = lane % I.n1;
       const int off = (line < lines ? line : 0) * (I.n1 + 2 * QL_HALO) + QL_HALO;
       // p-cell of this lane's line and its (j1, j2, j3), carried without
-      // divisions as the task advances by ``lines`` p-cells
-      const int p_first = tile.p0 + (line < lines ? line : 0);
+      // divisions as the task advances by ``lines`` p-cells; the two input
+      // slots alternate as in ens_task
+      const bool inline_ = line < lines;
+      const int p_first = tile.p0 + (inline_ ? line : 0);
       int j1 = p_first % I.n_p1, j23 = p_first / I.n_p1;
       int j2 = j23 % I.n_p2, j3 = j23 / I.n_p2;
-      T cur[1], nxt[1];
-      ens_load_line<T, 1>(ea, I, p_first, c, line < lines && line < tile.np, cur);
-      for (int q = 0; q < tile.np; q += lines) {
-        const bool live = line < lines && q + line < tile.np;
-        const bool more = line < lines && q + lines + line < tile.np;
-        ens_load_line<T, 1>(ea, I, more ? tile.p0 + q + lines + line : tile.p0, c, more, nxt);
-        ens_lines<T, R, 1>(ea, I, live ? tile.p0 + q + line : tile.p0, j1, j2, j3, c, live, cur,
-                           fl[wave] + off, kl[wave] + off, sl[wave] + off);
-        cur[0] = nxt[0];
-        j1 += lines;
-        while (j1 >= I.n_p1) {
-          j1 -= I.n_p1;
-          if (++j2 == I.n_p2) {
-            j2 = 0;
-            ++j3;
-          }
-        }
+      T* f_ = fl[wave] + off;
+      T* k_ = kl[wave] + off;
+      T* s_ = sl[wave] + off;
+      EnsLineIn<T, 1> A, B;
+      ens_fetch<T, 1>(ea, I, p_first, j1, j2, j3, c, inline_ && line < tile.np, A);
+      ens_advance(I, lines, j1, j2, j3);
+      for (int q = 0; q < tile.np;) {
+        bool live = inline_ && q + line < tile.np;
+        bool more = inline_ && q + lines + line < tile.np;
+        ens_fetch<T, 1>(ea, I, more ? tile.p0 + q + lines + line : tile.p0, j1, j2, j3, c, more, B);
+        ens_advance(I, lines, j1, j2, j3);
+        ens_lines<T, R, 1>(ea, I, live ? tile.p0 + q + line : tile.p0, c, live, A, f_, k_, s_);
+        q += lines;
+        if (q >= tile.np) break;
+        live = inline_ && q + line < tile.np;
+        more = inline_ && q + lines + line < tile.np;
+        ens_fetch<T, 1>(ea, I, more ? tile.p0 + q + lines + line : tile.p0, j1, j2, j3, c, more, A);
+        ens_advance(I, lines, j1, j2, j3);
+        ens_lines<T, R, 1>(ea, I, live ? tile.p0 + q + line : tile.p0, c, live, B, f_, k_, s_);
+        q += lines;
       }
     } else if (I.n1 <= 128) {
       ens_task<T, R, 2>(ea, I, tile, lane, fl[wave] + QL_HALO, kl[wave] + QL_HALO,
