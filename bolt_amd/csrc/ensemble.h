@@ -20,6 +20,7 @@
 // parameters, workgroup ranges balanced by cell count), so no kernel waits
 // on another workgroup.
 #pragma once
+#include <type_traits>
 #include "qline.h"
 
 namespace bx {
@@ -269,7 +270,7 @@ __global__ __launch_bounds__(256) void ens_stage_kernel(EnsStageArgs ea) {
 
 // Partial moments of one (instance, p-chunk): the chunk is a contiguous
 // span of (p1 - p0) * n1 elements, streamed in 16-byte vectors (non-temporal,
-// four in flight) when n1 is a multiple of the vector width, each thread
+// four in flight per thread) when n1 is a multiple of the vector width, each thread
 // keeping the same VEC q1 cells; otherwise one element per thread.  The
 // partial sums of threads holding the same cells are added in a fixed order.
 template <typename T>
@@ -297,37 +298,55 @@ __global__ __launch_bounds__(256) void ens_moments_kernel(const EnsInst* inst,
     const int pf = ch.p0 + g;
     int j1 = pf % I.n_p1, j23 = pf / I.n_p1;
     int j2 = j23 % I.n_p2, j3 = j23 / I.n_p2;
-    for (int p = pf; p < ch.p1; p += groups) {
-      const double v1 = centre(I.p1s, j1, I.dp1), v2 = centre(I.p2s, j2, I.dp2);
-      const double v3 = centre(I.p3s, j3, I.dp3);
-      const double e = 0.5 * ((v1 * v1 + v2 * v2) + v3 * v3);
-      double x[VEC];
-      if (vec) {
-        const V xv = __builtin_nontemporal_load((const V*)(col + (long)p * n1));
+    // four p-cells per pass: their loads are issued together, unconditionally
+    // (an index past the chunk reads p-cell pf and is then dropped), and the
+    // sums still run in increasing p
+    auto pass = [&](auto vec_c) {
+      constexpr bool VV = decltype(vec_c)::value;
+      for (int p = pf; p < ch.p1; p += 4 * groups) {
+        double x[4][VEC];
 #pragma unroll
-        for (int c = 0; c < VEC; ++c) x[c] = double(xv[c]);
-      } else {
-        x[0] = double(col[(long)p * n1]);
+        for (int u = 0; u < 4; ++u) {
+          const int pu = p + u * groups;
+          const long row = (long)(pu < ch.p1 ? pu : pf) * n1;
+          if constexpr (VV) {
+            const V xv = __builtin_nontemporal_load((const V*)(col + row));
 #pragma unroll
-        for (int c = 1; c < VEC; ++c) x[c] = 0.0;
-      }
+            for (int c = 0; c < VEC; ++c) x[u][c] = double(xv[c]);
+          } else {
+            x[u][0] = double(col[row]);
 #pragma unroll
-      for (int c = 0; c < VEC; ++c) {
-        a[0][c] += x[c];
-        a[1][c] += x[c] * v1;
-        a[2][c] += x[c] * v2;
-        a[3][c] += x[c] * v3;
-        a[4][c] += x[c] * e;
-      }
-      j1 += groups;
-      while (j1 >= I.n_p1) {
-        j1 -= I.n_p1;
-        if (++j2 == I.n_p2) {
-          j2 = 0;
-          ++j3;
+            for (int c = 1; c < VEC; ++c) x[u][c] = 0.0;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (p + u * groups < ch.p1) {
+            const double v1 = centre(I.p1s, j1, I.dp1), v2 = centre(I.p2s, j2, I.dp2);
+            const double v3 = centre(I.p3s, j3, I.dp3);
+            const double e = 0.5 * ((v1 * v1 + v2 * v2) + v3 * v3);
+#pragma unroll
+            for (int c = 0; c < VEC; ++c) {
+              a[0][c] += x[u][c];
+              a[1][c] += x[u][c] * v1;
+              a[2][c] += x[u][c] * v2;
+              a[3][c] += x[u][c] * v3;
+              a[4][c] += x[u][c] * e;
+            }
+          }
+          j1 += groups;
+          while (j1 >= I.n_p1) {
+            j1 -= I.n_p1;
+            if (++j2 == I.n_p2) {
+              j2 = 0;
+              ++j3;
+            }
+          }
         }
       }
-    }
+    };
+    if (vec) pass(std::integral_constant<bool, true>());
+    else pass(std::integral_constant<bool, false>());
   }
   __shared__ double cell[5][256];
   // per thread the partial of each of its cells, then a fixed-order sum over
