@@ -191,7 +191,7 @@ class Ensemble:
             if t < best_t:
                 best, best_t = cand, t
         # refine the moment chunking with the best shape
-        for chunk in (4096, 65536):
+        for chunk in (4096, 32768, 65536, 131072):
             cand = best[:3] + (chunk,)
             self.f = saved_f.clone()
             self._set_tuning(cand)
