@@ -203,31 +203,58 @@ __host__ __device__ inline void adj_reduce_cell(const AdjParams& ap, int q) {
   adj_reduce_finish(ap, q, r);
 }
 
-#ifndef BX_NO_KERNELS  // host-only builds of the per-cell code leave the kernels out
-constexpr int ADJ_RW = 8;  // waves of a reduction workgroup
+constexpr int ADJ_RW = 8;   // waves of a q1-line reduction workgroup
+constexpr int ADJ_PCH = 8;  // p-chunks of the q1-line reduction
 
-// q1-line layout: a workgroup takes 64 consecutive q1 cells of one q2 row
-// (lane = q1 cell, coalesced along the q1 lines) and its ADJ_RW waves take
-// interleaved p; the waves' sums are added in wave order.
-__global__ __launch_bounds__(64 * ADJ_RW) void adj_bgk_reduce_lines_kernel(AdjParams ap) {
+// The q1-line reduction splits p into ADJ_RW * ADJ_PCH interleaved streams:
+// stream s = c * ADJ_RW + w takes p = s, s + ADJ_RW * ADJ_PCH, ...  Chunk c
+// adds its streams in w order; the chunks are then added in c order.
+__host__ __device__ inline void adj_reduce_chunk(const AdjParams& ap, int q, int c, double r[6]) {
+  for (int k = 0; k < 6; ++k) r[k] = 0.0;
+  for (int w = 0; w < ADJ_RW; ++w) {
+    double x[6];
+    adj_reduce_sums(ap, q, c * ADJ_RW + w, ADJ_RW * ADJ_PCH, x);
+    for (int k = 0; k < 6; ++k) r[k] += x[k];
+  }
+}
+
+#ifndef BX_NO_KERNELS  // host-only builds of the per-cell code leave the kernels out
+// q1-line layout: workgroup (64 consecutive q1 cells of one q2 row, p-chunk
+// c = blockIdx.y): lane = q1 cell (coalesced along the q1 lines), wave w =
+// stream c * ADJ_RW + w; the waves' sums are added in wave order into
+// part[c][6][Q].
+__global__ __launch_bounds__(64 * ADJ_RW) void adj_bgk_reduce_lines_kernel(AdjParams ap,
+                                                                          double* part) {
   __shared__ double red[6][ADJ_RW][64];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int nblk = (ap.n1 + 63) / 64;
   const int i1 = (blockIdx.x % nblk) * 64 + lane, i2 = blockIdx.x / nblk;
+  const int c = blockIdx.y;
   const bool live = i1 < ap.n1;
+  const int Q = ap.n1 * ap.n2;
   const int q = i2 * ap.n1 + (live ? i1 : 0);
   double r[6] = {0, 0, 0, 0, 0, 0};
-  if (live) adj_reduce_sums(ap, q, wave, ADJ_RW, r);
+  if (live) adj_reduce_sums(ap, q, c * ADJ_RW + wave, ADJ_RW * ADJ_PCH, r);
   for (int k = 0; k < 6; ++k) red[k][wave][lane] = r[k];
   __syncthreads();
   if (wave == 0 && live) {
     for (int k = 0; k < 6; ++k) {
-      double x = red[k][0][lane];
-      for (int w = 1; w < ADJ_RW; ++w) x += red[k][w][lane];
-      r[k] = x;
+      double x = 0.0;
+      for (int w = 0; w < ADJ_RW; ++w) x += red[k][w][lane];
+      part[((long)c * 6 + k) * Q + q] = x;
     }
-    adj_reduce_finish(ap, q, r);
   }
+}
+
+// The chunks' sums of every q-cell added in chunk order, then the chain rule.
+__global__ __launch_bounds__(256) void adj_reduce_finish_kernel(AdjParams ap, const double* part) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Q = ap.n1 * ap.n2;
+  if (q >= Q) return;
+  double r[6] = {0, 0, 0, 0, 0, 0};
+  for (int c = 0; c < ADJ_PCH; ++c)
+    for (int k = 0; k < 6; ++k) r[k] += part[((long)c * 6 + k) * Q + q];
+  adj_reduce_finish(ap, q, r);
 }
 
 // Reference layout: a workgroup per q-cell, threads over p (contiguous),

@@ -878,7 +878,8 @@ extern "C" int boltx_check_finite(const boltx_grid* grid, const double* mom, int
 // ---- adjoint stage -----------------------------------------------------------
 extern "C" size_t boltx_adjoint_workspace_size(const boltx_grid* grid, const boltx_physics*) {
   if (!grid) return 0;
-  return (size_t)5 * grid->n_q1 * grid->n_q2 * sizeof(double);
+  // moment weights [5][Q], then the q1-line reduction's chunk sums [ADJ_PCH][6][Q]
+  return (size_t)(5 + 6 * ADJ_PCH) * grid->n_q1 * grid->n_q2 * sizeof(double);
 }
 
 extern "C" int boltx_adjoint_stage(const boltx_grid* grid, const boltx_physics* phys,
@@ -903,8 +904,13 @@ extern "C" int boltx_adjoint_stage(const boltx_grid* grid, const boltx_physics* 
   const long Q = (long)ap.n1 * ap.n2;
   if (ap.source) {
     if (ap.layout) {
+      double* part = ap.gq + 5 * Q;
       const unsigned nb1 = (unsigned)((ap.n1 + 63) / 64) * ap.n2;
-      hipLaunchKernelGGL(adj_bgk_reduce_lines_kernel, dim3(nb1), dim3(64 * ADJ_RW), 0, s, ap);
+      hipLaunchKernelGGL(adj_bgk_reduce_lines_kernel, dim3(nb1, ADJ_PCH), dim3(64 * ADJ_RW), 0, s,
+                         ap, part);
+      if ((rc = launch_status("adj_bgk_reduce_lines_kernel"))) return rc;
+      hipLaunchKernelGGL(adj_reduce_finish_kernel, dim3((unsigned)((Q + 255) / 256)), dim3(256), 0,
+                         s, ap, (const double*)part);
     } else {
       hipLaunchKernelGGL(adj_bgk_reduce_kernel, dim3((unsigned)Q), dim3(256), 0, s, ap);
     }

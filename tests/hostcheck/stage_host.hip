@@ -143,17 +143,24 @@ extern "C" int host_adjoint_stage(const boltx_grid* g, const boltx_physics* ph,
                                   const boltx_adjoint* adj) {
   const AdjParams ap = adj_params(g, ph, adj);
   const int Q = ap.n1 * ap.n2;
-  // the reduction over p split as the device kernels split it: interleaved
-  // p-ranges (8 in the q1-line layout, 256 in the reference layout) whose
-  // sums are added in order
-  const int split = ap.layout ? 8 : 256;
+  // the reduction over p split as the device kernels split it: in the
+  // q1-line layout ADJ_PCH chunks of ADJ_RW interleaved streams (chunks added
+  // in order), in the reference layout 256 interleaved streams added in order
   if (ap.source)
     for (int q = 0; q < Q; ++q) {
       double r[6] = {0, 0, 0, 0, 0, 0};
-      for (int w = 0; w < split; ++w) {
-        double x[6];
-        adj_reduce_sums(ap, q, w, split, x);
-        for (int k = 0; k < 6; ++k) r[k] += x[k];
+      if (ap.layout) {
+        for (int c = 0; c < ADJ_PCH; ++c) {
+          double x[6];
+          adj_reduce_chunk(ap, q, c, x);
+          for (int k = 0; k < 6; ++k) r[k] += x[k];
+        }
+      } else {
+        for (int w = 0; w < 256; ++w) {
+          double x[6];
+          adj_reduce_sums(ap, q, w, 256, x);
+          for (int k = 0; k < 6; ++k) r[k] += x[k];
+        }
       }
       adj_reduce_finish(ap, q, r);
     }

@@ -2,6 +2,7 @@
 products against autograd through the FP64 oracle, the checkpointed
 whole-run gradient against autograd through plain-torch oracle steps, and
 the adjoint workload's gradients against central finite differences."""
+import ctypes
 import math
 
 import numpy as np
@@ -27,6 +28,10 @@ def _oracle_vjp(o, u_int, mu_int, tau_value):
     return gu[:, 0], float(gt)
 
 
+def _ws_bytes(s):
+    return int(s._L.boltx_adjoint_workspace_size(ctypes.byref(s._grid), ctypes.byref(s._phys)))
+
+
 @pytest.mark.parametrize("layout", ["reference", "q1-lines"])
 @pytest.mark.parametrize("rq", ["minmod", "ppm", "weno5"])
 def test_stage_adjoint_vs_oracle_autograd(gpu, layout, rq):
@@ -43,7 +48,7 @@ def test_stage_adjoint_vs_oracle_autograd(gpu, layout, rq):
     U = A.interior_to_internal(s, u[:, 0].to(gpu))
     MU = A.interior_to_internal(s, mu[:, 0].to(gpu))
     mom = A._moments_of(s, U)
-    ws = torch.empty(5 * s._Q * 8, dtype=torch.uint8, device=gpu)
+    ws = torch.empty(_ws_bytes(s), dtype=torch.uint8, device=gpu)
     tq = torch.empty(s._Q, dtype=F64, device=gpu)
     out = A._adjoint_stage(s, U, mom, MU, None, 0.0, 0.0, 1.0, ws, tq)
     got = A.internal_to_interior(s, out).cpu()
@@ -72,7 +77,7 @@ def test_stage_adjoint_ragged_reduction(gpu, layout, n1, np1, np2):
     U = A.interior_to_internal(s, u[:, 0].to(gpu))
     MU = A.interior_to_internal(s, mu[:, 0].to(gpu))
     mom = A._moments_of(s, U)
-    ws = torch.empty(5 * s._Q * 8, dtype=torch.uint8, device=gpu)
+    ws = torch.empty(_ws_bytes(s), dtype=torch.uint8, device=gpu)
     tq = torch.empty(s._Q, dtype=F64, device=gpu)
     out = A._adjoint_stage(s, U, mom, MU, None, 0.0, 0.0, 1.0, ws, tq)
     got = A.internal_to_interior(s, out).cpu()

@@ -74,7 +74,7 @@ def test_adjoint_and_ensemble_reject_bad_input():
     adj.out = adj.lam
     assert lib.boltx_adjoint_stage(C.byref(g), C.byref(ph), C.byref(adj), None) == -1
     assert b"alias" in lib.boltx_last_error()
-    assert lib.boltx_adjoint_workspace_size(C.byref(g), C.byref(ph)) == 5 * 64 * 8
+    assert lib.boltx_adjoint_workspace_size(C.byref(g), C.byref(ph)) == (5 + 6 * 8) * 64 * 8
     m = (_lib.EnsMember * 1)()
     m[0].grid = g
     m[0].grid.n_q2 = 1
