@@ -28,7 +28,7 @@ def _members(precision, sizes, stepper="SSP-RK3", seed=0):
     return out
 
 
-SIZES = [(32, 8), (40, 6), (64, 10), (128, 6), (256, 4), (48, 12)]
+SIZES = [(32, 8), (40, 6), (64, 10), (128, 6), (256, 4), (48, 12), (37, 6), (16, 8)]
 
 
 def _rel_linf(a, b):
