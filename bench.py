@@ -313,9 +313,16 @@ def main():
     ap.add_argument("--probe", choices=["fp64", "fp32-bulk"],
                     help="traffic probe for rocprofv3 --pmc: a 256 MiB device copy "
                          "(the counter calibration) and then one step of the workload")
+    ap.add_argument("--only", choices=["em_2d3v", "adjoint", "ensemble"],
+                    help="run one of the other workloads alone (for profiling) and print "
+                         "its JSON object")
     args = ap.parse_args()
     if args.probe:
         probe(args.probe)
+        return
+    if args.only:
+        run = {"em_2d3v": run_em, "adjoint": run_adjoint, "ensemble": run_ensemble}[args.only]
+        print(json.dumps({args.only: run()}), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
