@@ -175,7 +175,8 @@ int boltx_yee_to_cell_centered(const boltx_grid *grid, const double *yee,
 int boltx_check_finite(const boltx_grid *grid, const double *mom, int32_t *flag,
                        void *stream);
 
-/* Adjoint of one stage (FP64, fields off).  For the stage
+/* Adjoint of one stage (FP64, fields off, periodic q: a grid with ghost
+ * q-rows is rejected with BOLTX_EUNSUPPORTED).  For the stage
  * out = a*un + b*(w + cdt*L(u)) and an adjoint lam of its output, the caller
  * passes c = b*cdt and gets out = a*base + b*lam + c*J(u)^T lam, J = dL/du,
  * and, if tau_grad is non-NULL, tau_grad[q] = c * sum_p lam * dL/dtau for
