@@ -74,6 +74,11 @@ def test_adjoint_and_ensemble_reject_bad_input():
     adj.out = adj.lam
     assert lib.boltx_adjoint_stage(C.byref(g), C.byref(ph), C.byref(adj), None) == -1
     assert b"alias" in lib.boltx_last_error()
+    adj.out, adj.mom_u, adj.workspace = C.c_void_p(48), C.c_void_p(64), C.c_void_p(80)
+    g.n_ghost, g.bc[0], g.bc[1] = 3, _lib.BC["mirror"], _lib.BC["mirror"]
+    assert lib.boltx_adjoint_stage(C.byref(g), C.byref(ph), C.byref(adj), None) == -3
+    assert b"periodic" in lib.boltx_last_error()
+    g.n_ghost, g.bc[0], g.bc[1] = 0, 0, 0
     assert lib.boltx_adjoint_workspace_size(C.byref(g), C.byref(ph)) == (5 + 6 * 8) * 64 * 8
     m = (_lib.EnsMember * 1)()
     m[0].grid = g
