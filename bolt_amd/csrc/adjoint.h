@@ -44,7 +44,7 @@ __host__ __device__ inline Dual operator*(Dual a, Dual b) {
   return Dual(a.v * b.v, a.d * b.v + a.v * b.d);
 }
 __host__ __device__ inline Dual operator/(Dual a, Dual b) {
-  const double r = 1.0 / b.v;
+  const double r = fast_rcp(b.v);
   return Dual(a.v * r, (a.d - a.v * r * b.d) * r);
 }
 __host__ __device__ inline bool operator<(Dual a, Dual b) { return a.v < b.v; }
@@ -53,7 +53,7 @@ __host__ __device__ inline bool operator<=(Dual a, Dual b) { return a.v <= b.v; 
 __host__ __device__ inline bool operator>=(Dual a, Dual b) { return a.v >= b.v; }
 __host__ __device__ inline Dual aabs(Dual a) { return a.v < 0.0 ? -a : a; }
 __host__ __device__ inline Dual fast_rcp(Dual a) {
-  const double r = 1.0 / a.v;
+  const double r = fast_rcp(a.v);
   return Dual(r, -a.d * r * r);
 }
 // weno5.py:58-71 (expanded indicator plus the eps floor) on duals
