@@ -482,8 +482,16 @@ static int launch_qc_t(const QCParams& qp, hipStream_t s) {
   size_t lds = sizeof(T) * (size_t)(sp.n_p1 + sp.n_p2 + sp.n_p3);
   if (PADV) lds += sizeof(T) * (size_t)sp.P;
   if (lds > 160 * 1024) return fail(BOLTX_EUNSUPPORTED, "p-block of %zu bytes exceeds LDS", lds);
-  hipLaunchKernelGGL((qcell_kernel<T, RQ, RP, PADV>), dim3(sp.Q), dim3(qc_block<T, PADV>()), lds, s,
-                     qp);
+  if constexpr (!PADV) {
+    hipLaunchKernelGGL((qcell_kernel<T, RQ, RP, false, 256>), dim3(sp.Q), dim3(256), lds, s, qp);
+  } else {
+    if (sizeof(T) == 4 && 2 * (lds + 1024) <= 160 * 1024) {
+      // two 8-wave workgroups per CU (the 1024 B cover the kernel's own LDS)
+      hipLaunchKernelGGL((qcell_kernel<T, RQ, RP, true, 512>), dim3(sp.Q), dim3(512), lds, s, qp);
+    } else {
+      hipLaunchKernelGGL((qcell_kernel<T, RQ, RP, true, 1024>), dim3(sp.Q), dim3(1024), lds, s, qp);
+    }
+  }
   return launch_status("qcell_kernel");
 }
 

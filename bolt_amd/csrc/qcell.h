@@ -114,13 +114,12 @@ struct QCParams {
   int nslab;
 };
 
-// Workgroup size: with p-advection the whole p-block of the q-cell sits in
-// LDS.  An FP64 block (110 KB at 24^3 p-cells) leaves room for one workgroup
-// per CU, which then runs 16 waves; an FP32 block is half that, so two
-// workgroups of 8 waves share a CU and one fills its block while the other
-// computes.  Without p-advection the workgroup is 4 waves.
-template <typename T, bool PADV>
-constexpr int qc_block() { return PADV ? (sizeof(T) == 4 ? 512 : 1024) : 256; }
+// Workgroup size (threads NT, a template parameter): with p-advection the
+// whole p-block of the q-cell sits in LDS.  An FP64 block (110 KB at 24^3
+// p-cells) leaves room for one workgroup per CU, which then runs 16 waves; an
+// FP32 block that fits twice into the 160 KB of a CU runs as two workgroups of
+// 8 waves, so one fills its block while the other computes (the launch picks
+// NT).  Without p-advection the workgroup is 4 waves.
 
 // q-cell of workgroup b of nb = Q: the eight XCDs take contiguous bands of
 // q2 rows, and each walks its band q2-fastest, so a q-cell's stencil
@@ -134,9 +133,9 @@ __device__ inline int qc_order(const StageParams& sp, int b, int nb) {
   return (l / rows) + sp.n_q1 * (x * rows + l % rows);
 }
 
-template <typename T, int RQ, int RP, bool PADV>
-__global__ __launch_bounds__((qc_block<T, PADV>())) void qcell_kernel(QCParams qp) {
-  constexpr int NW = qc_block<T, PADV>() / 64;
+template <typename T, int RQ, int RP, bool PADV, int NT>
+__global__ __launch_bounds__(NT) void qcell_kernel(QCParams qp) {
+  constexpr int NW = NT / 64;
   extern __shared__ unsigned char smem[];
   __shared__ double red[5][NW];
   const StageParams& sp = qp.sp;

@@ -162,6 +162,20 @@ def test_2d3v_fields_fp32_bulk_vs_fp64_oracle(gpu):
     assert np.abs(a - b).max() <= 1e-5 * max(np.abs(b).max(), 1e-30)
 
 
+def test_2d3v_fields_fp32_bulk_large_p_block(gpu):
+    """An FP32 p-block too large for two workgroups per CU (28^3 p-cells,
+    88 KB) runs as one 16-wave workgroup per q-cell; against the FP64 oracle."""
+    c64 = case_2d3v(seed=7, N_q=4, N_p=28)
+    c32 = case_2d3v(seed=7, N_q=4, N_p=28, precision="fp32-bulk")
+    o, nls = oracle_solver(c64), hip_solver(c32)
+    for _ in range(3):
+        o.strang_timestep(c64.dt)
+        nls.strang_timestep(c32.dt)
+    linf, l2 = _cmp(nls, o)
+    print(f"2D3V FP32-bulk, 28^3 p-cells, 3 steps vs FP64 oracle: rel L2 {l2:.3e}", flush=True)
+    assert l2 <= 1e-5
+
+
 def test_divergence_raises(gpu):
     c = case_c1(seed=0)
     nls = hip_solver(c)
