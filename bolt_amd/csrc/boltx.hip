@@ -485,7 +485,7 @@ static int launch_qc_t(const QCParams& qp, hipStream_t s) {
   if constexpr (!PADV) {
     hipLaunchKernelGGL((qcell_kernel<T, RQ, RP, false, 256>), dim3(sp.Q), dim3(256), lds, s, qp);
   } else {
-    if (sizeof(T) == 4 && 2 * (lds + 1024) <= 160 * 1024) {
+    if (2 * (lds + 1024) <= 160 * 1024) {
       // two 8-wave workgroups per CU (the 1024 B cover the kernel's own LDS)
       hipLaunchKernelGGL((qcell_kernel<T, RQ, RP, true, 512>), dim3(sp.Q), dim3(512), lds, s, qp);
     } else {

@@ -115,11 +115,11 @@ struct QCParams {
 };
 
 // Workgroup size (threads NT, a template parameter): with p-advection the
-// whole p-block of the q-cell sits in LDS.  An FP64 block (110 KB at 24^3
-// p-cells) leaves room for one workgroup per CU, which then runs 16 waves; an
-// FP32 block that fits twice into the 160 KB of a CU runs as two workgroups of
-// 8 waves, so one fills its block while the other computes (the launch picks
-// NT).  Without p-advection the workgroup is 4 waves.
+// whole p-block of the q-cell sits in LDS.  A block that fits twice into the
+// 160 KB of a CU (FP32 at 24^3 p-cells: 55 KB) runs as two workgroups of 8
+// waves, so one fills its block while the other computes; a larger one (FP64
+// at 24^3: 110 KB) as one workgroup of 16 waves (the launch picks NT).
+// Without p-advection the workgroup is 4 waves.
 
 // q-cell of workgroup b of nb = Q: the eight XCDs take contiguous bands of
 // q2 rows, and each walks its band q2-fastest, so a q-cell's stencil

@@ -176,6 +176,13 @@ def test_2d3v_fields_fp32_bulk_large_p_block(gpu):
     assert l2 <= 1e-5
 
 
+def test_2d3v_fields_large_p_block_fp64(gpu):
+    """An FP64 p-block too large for two workgroups per CU (22^3 p-cells,
+    83 KB) runs as one 16-wave workgroup per q-cell: per-step parity."""
+    c = case_2d3v(seed=8, N_q=4, N_p=22)
+    _per_step(c, 2, 1e-12)
+
+
 def test_divergence_raises(gpu):
     c = case_c1(seed=0)
     nls = hip_solver(c)
