@@ -160,29 +160,45 @@ __global__ __launch_bounds__(kBlock) void moments_kernel(StageParams sp) {
 // Slab order of the p-cells for qcell_kernel: grouped by the signs of
 // (v1, v2) -- (<=0,<=0), (>0,<=0), (<=0,>0), (>0,>0) -- each group in natural
 // order and padded with -1 to a multiple of 64; jpack holds j1 | j2<<10 | j3<<20
-// of each entry (0 for padding).  One thread; run once per problem.
-__global__ void slab_perm_kernel(int* perm, int* jpack, int n_p1, int n_p2, int n_p3, double p1s,
-                                 double dp1, double p2s, double dp2) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// of each entry (0 for padding).  One wave; 64 p-cells per step are tested
+// together and the members of the group are packed in order by a ballot.
+// Run once per problem.
+__global__ __launch_bounds__(64) void slab_perm_kernel(int* perm, int* jpack, int n_p1, int n_p2,
+                                                       int n_p3, double p1s, double dp1, double p2s,
+                                                       double dp2) {
+  if (blockIdx.x != 0) return;
+  const int lane = threadIdx.x;
   const int P = n_p1 * n_p2 * n_p3;
-  int k = 0;
+  int k = 0;  // entries written so far (uniform across the wave)
   for (int grp = 0; grp < 4; ++grp) {
     const bool pos1 = grp & 1, pos2 = grp >> 1;
     const int start = k;
-    for (int p = 0; p < P; ++p) {
-      const int j1 = p % n_p1, j2 = (p / n_p1) % n_p2, j3 = p / (n_p1 * n_p2);
-      if ((centre(p1s, j1, dp1) > 0.0) == pos1 && (centre(p2s, j2, dp2) > 0.0) == pos2) {
-        perm[k] = p;
-        jpack[k] = j1 | (j2 << 10) | (j3 << 20);
-        ++k;
+    for (int base = 0; base < P; base += 64) {
+      const int p = base + lane;
+      int j1 = 0, j2 = 0, j3 = 0;
+      bool in = false;
+      if (p < P) {
+        j1 = p % n_p1;
+        j2 = (p / n_p1) % n_p2;
+        j3 = p / (n_p1 * n_p2);
+        in = (centre(p1s, j1, dp1) > 0.0) == pos1 && (centre(p2s, j2, dp2) > 0.0) == pos2;
       }
+      const unsigned long long mask = __ballot(in);
+      if (in) {
+        const int at = k + __popcll(mask & ((1ull << lane) - 1ull));
+        perm[at] = p;
+        jpack[at] = j1 | (j2 << 10) | (j3 << 20);
+      }
+      k += __popcll(mask);
     }
-    if (k > start)
-      while (k % 64 != 0) {
-        perm[k] = -1;
-        jpack[k] = 0;
-        ++k;
+    if (k > start) {
+      const int pad = (64 - k % 64) % 64;
+      if (lane < pad) {
+        perm[k + lane] = -1;
+        jpack[k + lane] = 0;
       }
+      k += pad;
+    }
   }
 }
 
