@@ -606,10 +606,10 @@ static int ql_moments(const boltx_grid* g, const StageParams& sp, const void* f,
 
 template <typename T>
 static int ql_stage(const boltx_grid* g, const boltx_physics* ph, const boltx_stage* st,
-                    const StageParams& sp, hipStream_t s) {
+                    void* workspace, const StageParams& sp, hipStream_t s) {
   if (g->n_q1 > 1 && g->n_q1 < QL_HALO)
     return fail(BOLTX_EUNSUPPORTED, "the q1-line layout needs n_q1 = 1 or n_q1 >= %d", QL_HALO);
-  char* ws = (char*)st->workspace;
+  char* ws = (char*)workspace;
   T* tab = (T*)ws;
   double* part = (double*)(ws + ql_table_bytes(g, ph->precision));
   int rc;
@@ -683,8 +683,8 @@ extern "C" int boltx_version(void) { return BOLTX_VERSION; }
 
 extern "C" const char* boltx_last_error(void) { return g_err; }
 
-extern "C" int boltx_fvm_stage(const boltx_grid* grid, const boltx_physics* phys,
-                               const boltx_stage* st, void* stream) {
+// Argument checks of one stage (everything but the workspace).
+static int check_stage(const boltx_grid* grid, const boltx_physics* phys, const boltx_stage* st) {
   int rc = check_grid(grid);
   if (rc) return rc;
   if ((rc = check_physics(phys))) return rc;
@@ -701,6 +701,12 @@ extern "C" int boltx_fvm_stage(const boltx_grid* grid, const boltx_physics* phys
     return fail(BOLTX_EINVAL, "n_ghost = %d is below the stencil reach %d", grid->n_ghost, need);
   if (ghost_rows(grid) > 0 && phys->p_advection)
     return fail(BOLTX_EUNSUPPORTED, "non-periodic q boundaries with p-advection");
+  return BOLTX_OK;
+}
+
+// One stage with the given workspace (the caller's, or the scratch array).
+static int fvm_stage_impl(const boltx_grid* grid, const boltx_physics* phys,
+                          const boltx_stage* st, void* workspace, hipStream_t s) {
   StageParams sp = base_params(grid);
   sp.riemann_q = phys->riemann_q;
   sp.riemann_p = phys->riemann_p;
@@ -722,19 +728,18 @@ extern "C" int boltx_fvm_stage(const boltx_grid* grid, const boltx_physics* phys
   sp.fp1 = st->fields_c_p1;
   sp.fp2 = st->fields_c_p2;
   sp.fp3 = st->fields_c_p3;
-  hipStream_t s = (hipStream_t)stream;
   const bool padv = phys->p_advection != 0;
   if (grid->layout == BOLTX_LAYOUT_Q1_LINES) {
     if (padv) return fail(BOLTX_EUNSUPPORTED, "p-advection needs the reference layout");
-    if (!st->workspace) return fail(BOLTX_EINVAL, "the q1-line layout needs the workspace");
-    if (phys->precision == BOLTX_FP64) return ql_stage<double>(grid, phys, st, sp, s);
-    return ql_stage<float>(grid, phys, st, sp, s);
+    if (!workspace) return fail(BOLTX_EINVAL, "the q1-line layout needs the workspace");
+    if (phys->precision == BOLTX_FP64) return ql_stage<double>(grid, phys, st, workspace, sp, s);
+    return ql_stage<float>(grid, phys, st, workspace, sp, s);
   }
-  if (st->workspace && qc_applies(grid)) {
+  if (workspace && qc_applies(grid)) {
     QCParams qp;
     qp.sp = sp;
     qp.nslab = qc_nslab(grid);
-    qp.perm = (const int*)st->workspace;
+    qp.perm = (const int*)workspace;
     qp.jpack = qp.perm + (size_t)qp.nslab * 64;
     if (phys->precision == BOLTX_FP64)
       return launch_qc<double>(qp, phys->recon_q, phys->recon_p, padv, s);
@@ -743,6 +748,13 @@ extern "C" int boltx_fvm_stage(const boltx_grid* grid, const boltx_physics* phys
   if (phys->precision == BOLTX_FP64)
     return launch_stage_rq<double>(sp, phys->recon_q, phys->recon_p, padv, s);
   return launch_stage_rq<float>(sp, phys->recon_q, phys->recon_p, padv, s);
+}
+
+extern "C" int boltx_fvm_stage(const boltx_grid* grid, const boltx_physics* phys,
+                               const boltx_stage* st, void* stream) {
+  int rc = check_stage(grid, phys, st);
+  if (rc) return rc;
+  return fvm_stage_impl(grid, phys, st, st->workspace, (hipStream_t)stream);
 }
 
 extern "C" int boltx_moments(const boltx_grid* grid, const boltx_physics* phys, const void* f,
@@ -924,6 +936,10 @@ extern "C" int boltx_adjoint_stage(const boltx_grid* grid, const boltx_physics* 
   if (ghost_rows(grid) > 0)
     return fail(BOLTX_EUNSUPPORTED, "the adjoint needs periodic q boundaries");
   const AdjParams ap = adj_params(grid, phys, adj);
+  // every limit is checked before the first launch: a rejected call queues
+  // no device work
+  if ((long)ap.n1 * ap.P >= (1L << 31) || ap.n2 > 65535)
+    return fail(BOLTX_EUNSUPPORTED, "the adjoint grid takes n_q1*P < 2^31 and n_q2 <= 65535");
   hipStream_t s = (hipStream_t)stream;
   const long Q = (long)ap.n1 * ap.n2;
   if (ap.source) {
@@ -940,8 +956,6 @@ extern "C" int boltx_adjoint_stage(const boltx_grid* grid, const boltx_physics* 
     }
     if ((rc = launch_status("adj_bgk_reduce_kernel"))) return rc;
   }
-  if ((long)ap.n1 * ap.P >= (1L << 31) || ap.n2 > 65535)
-    return fail(BOLTX_EUNSUPPORTED, "the adjoint grid takes n_q1*P < 2^31 and n_q2 <= 65535");
   const dim3 nb((unsigned)(((long)ap.n1 * ap.P + 255) / 256), (unsigned)ap.n2);
   switch (phys->recon_q) {
     case RC_PC: hipLaunchKernelGGL(adj_stage_kernel<RC_PC>, nb, dim3(256), 0, s, ap); break;

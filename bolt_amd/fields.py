@@ -2,13 +2,21 @@
 bolt/lib/nonlinear/fields/fields.py:19-690 (FDTD branch) on the GPU.
 
 Field arrays are FP64 [6][Q] device tensors (E1, E2, E3, B1, B2, B3; Q interior
-q-cells, periodic, no ghost zones).  The staggered bookkeeping of
-evolve_electrodynamic_fields (fields.py:568-647) is kept as in the reference,
-including fields.py:493-494, which binds ``cell_centered_EM_fields_at_n`` and
-``cell_centered_EM_fields_at_n_plus_half`` to the *same* array as
-``cell_centered_EM_fields`` (component writes through any of the three names
-change all of them).  The FDTD update and the Yee-to-centre averaging run in
-the HIP library.
+q-cells, periodic, no ghost zones).  The FDTD update and the Yee-to-centre
+averaging run in the HIP library.  Two couplings to the RK stages:
+
+- 'toggle' (RK2, the reference's stepper): the staggered bookkeeping of
+  evolve_electrodynamic_fields (fields.py:568-647) as in the reference, run
+  from the RHS whenever at_n is False, with at_n flipped by every C_p
+  evaluation (fields.py:685-688), and fields.py:493-494, which binds
+  ``cell_centered_EM_fields_at_n`` and ``..._at_n_plus_half`` to the *same*
+  array as ``cell_centered_EM_fields`` (component writes through any of the
+  three names change all of them);
+- 'once-per-step' (SSP-RK3, which the reference does not have: its toggle
+  would run FDTD on 1, 2, 1, 2, ... of the stages and advance the fields
+  1.5 dt per step): the stages sit at t^n, t^{n+1}, t^{n+1/2} and see the
+  fields at n, n+1 and n+1/2; one leapfrog per step, between stages 1 and 2,
+  with J^{n+1/2} = (J(u^n) + J(u1)) / 2, and separate arrays per time level.
 """
 import ctypes
 import math
@@ -52,10 +60,18 @@ class fields_solver(object):
                 cc[3:] = tmp[3:]
         else:
             raise NotImplementedError('Method not valid/not implemented')
-        # fields.py:493-494: all three names refer to one object.
         self._cc = cc
-        self._cc_at_n = cc
-        self._cc_at_n_plus_half = cc
+        rk3 = getattr(p, 'time_stepper_fvm', 'RK2') == 'SSP-RK3'
+        self.coupling = 'once-per-step' if rk3 and self._fdtd else 'toggle'
+        self.stage_fields = None
+        if self.coupling == 'toggle':
+            # fields.py:493-494: all three names refer to one object.
+            self._cc_at_n = cc
+            self._cc_at_n_plus_half = cc
+        else:
+            # separate time levels; at t = 0, B^{-1/2} = B^{1/2} (fields.py:491)
+            self._cc_at_n = cc.clone()
+            self._cc_at_n_plus_half = cc.clone()
 
     # -- initialisation (fields.py:280-419) ----------------------------------
     def _eval_interior(self, fn, q1, q2, k):
@@ -140,9 +156,31 @@ class fields_solver(object):
         self._cc_at_n_plus_half[:3] = 0.5 * (E_at_n + E_at_n_plus_one)
         self.time_elapsed += dt
 
+    def evolve_once_per_step(self, J, dt):
+        """The once-per-step leapfrog of SSP-RK3 (J = J^{n+1/2}, [3][Q] FP64
+        summed over species): from (E^n, B^{n+1/2}) to (E^{n+1}, B^{n+3/2})
+        with the time levels of fields.py:568-647 in separate arrays:
+        ``_cc_at_n`` = (E^{n+1}, (B^{n+1/2} + B^{n+3/2}) / 2) for stage 2 and the
+        next step's stage 1, ``_cc_at_n_plus_half`` = ((E^n + E^{n+1}) / 2,
+        B^{n+1/2}) for stage 3."""
+        s = self._solver
+        E_n = self._cc[:3].clone()
+        B_n_plus_half = self._cc[3:].clone()
+        _lib.check(s._L.boltx_fdtd_step(ctypes.byref(s._grid), _lib.ptr(self._yee), _lib.ptr(J),
+                                        float(dt), float(self.params.eps), float(self.params.mu),
+                                        _lib.stream_handle()))
+        self._yee_to_cc(self._cc)
+        self._cc_at_n = torch.cat([self._cc[:3], 0.5 * (B_n_plus_half + self._cc[3:])])
+        self._cc_at_n_plus_half = torch.cat([0.5 * (E_n + self._cc[:3]), B_n_plus_half])
+        self.time_elapsed += dt
+
     def _take_for_C_p(self):
-        """The [6][Q] array one C_p evaluation sees, then the toggle of
-        fields.py:685-688."""
+        """The [6][Q] array one C_p evaluation sees: with the toggle coupling
+        the time level at_n selects, then the toggle of fields.py:685-688;
+        with the once-per-step coupling the stage's fields (the fields at n
+        outside a step), no toggle."""
+        if self.coupling == 'once-per-step':
+            return self.stage_fields if self.stage_fields is not None else self._cc_at_n
         if not self._fdtd:
             src = self._cc
         else:

@@ -515,28 +515,53 @@ class nonlinear_solver(object):
         return t
 
     # -- the step -------------------------------------------------------------
-    def _fields_for_stage(self, us):
+    def _currents(self, us, name):
+        """J ([3][Q] FP64) of the species' states ``us`` summed over species
+        (df_dt_fvm.py:214-224, fields.py:604-606), into work array ``name``."""
+        J = None
+        for k in self._species_loop():
+            Jk = self._buf('J', F64, (3, self._Q))
+            _lib.check(self._L.boltx_currents(ctypes.byref(self._grid),
+                                              ctypes.byref(self._phys), _lib.ptr(us[k]),
+                                              _lib.ptr(Jk), _lib.stream_handle()))
+            if J is None:
+                J = self._spc[0].bufs.setdefault(name, torch.empty_like(Jk))
+                J.copy_(Jk)
+            else:
+                J += Jk
+        return J
+
+    def _fields_for_stage(self, us, stage=None):
         """The field work of one RHS evaluation (df_dt_fvm.py:102-104,
-        214-267): the electrostatic solve from rho, or the FDTD update from J
-        when at_n is False, with J and rho summed over species (fields.py:562,
-        604-606); then the three field arrays C_p1..C_p3 see, toggling at_n.
-        ``us`` are the species' stage inputs; their moment blocks are ``moms``
-        of the stage (set by the caller)."""
+        214-267), with J and rho summed over species (fields.py:562,
+        604-606), and the three field arrays C_p1..C_p3 see.  ``us`` are the
+        species' stage inputs; their moment blocks are ``_stage_moms`` (set by
+        the caller).  Electrostatic: the solve from rho of the stage input
+        (df_dt_fvm.py:235-242).  Electrodynamic with the reference's toggle
+        (RK2): the FDTD update from J when at_n is False, then at_n toggles
+        per C_p.  Electrodynamic under SSP-RK3 (``stage`` 0, 1, 2): stage 1
+        sees the fields at n and keeps J(u^n); stage 2 takes J(u1), advances
+        the fields once with their average J^{n+1/2} and sees the fields at
+        n+1; stage 3 sees those at n+1/2."""
         F = [None, None, None]
         if not self._p_adv:
             return F
         fs = self.fields_solver
         if self._es:  # df_dt_fvm.py:235-242: rho of the stage input
             fs.compute_electrostatic_fields(self._rho(self._stage_moms))
+        elif fs.coupling == 'once-per-step':
+            if stage == 0:
+                self._currents(us, 'J_n')
+                fs.stage_fields = fs._cc_at_n
+            elif stage == 1:
+                J_n = self._spc[0].bufs['J_n']
+                J_1 = self._currents(us, 'J_1')
+                fs.evolve_once_per_step(0.5 * (J_n + J_1), self.dt)
+                fs.stage_fields = fs._cc_at_n
+            elif stage == 2:
+                fs.stage_fields = fs._cc_at_n_plus_half
         elif fs.at_n is False:
-            J = None
-            for k in self._species_loop():
-                Jk = self._buf('J', F64, (3, self._Q))
-                _lib.check(self._L.boltx_currents(ctypes.byref(self._grid),
-                                                  ctypes.byref(self._phys), _lib.ptr(us[k]),
-                                                  _lib.ptr(Jk), _lib.stream_handle()))
-                J = Jk if J is None else J + Jk
-            fs.evolve_electrodynamic_fields(J, self.dt)
+            fs.evolve_electrodynamic_fields(self._currents(us, 'J_sum'), self.dt)
         return [fs._take_for_C_p() for _ in range(3)]  # df_dt_fvm.py:251-267
 
     def _stage_kernel(self, u, w, un, out, a, b, cdt, mom_in, mom_out, F):
@@ -598,18 +623,20 @@ class nonlinear_solver(object):
                 return self._buf(name, F64, (_lib.MOM_ROWS, self._Q))
             return self._buf(name)
 
-        for (u, w, un, out, a, b, cdt, mi, mo) in plan:
+        for stage, (u, w, un, out, a, b, cdt, mi, mo) in enumerate(plan):
             args = []
             for _ in self._species_loop():
                 args.append([arr(x) for x in (u, w, un, out, mi, mo)])
             self._stage_moms = [x[4] for x in args]
-            F = self._fields_for_stage([x[0] for x in args])
+            F = self._fields_for_stage([x[0] for x in args], stage)
             for k in self._species_loop():
                 uu, ww, nn, oo, mmi, mmo = args[k]
                 self._stage_kernel(uu, ww, nn, oo, a, b, cdt, mmi, mmo, F)
         for _ in self._species_loop():
             _lib.check(self._L.boltx_check_finite(ctypes.byref(self._grid), _lib.ptr(self._mom),
                                                   _lib.ptr(self._flag), _lib.stream_handle()))
+        if self.fields_solver is not None:
+            self.fields_solver.stage_fields = None
         self.time_elapsed += dt
 
     def check_divergence(self):

@@ -116,7 +116,11 @@ typedef struct boltx_stage {
  * reconstruction and upwind / LLF fluxes, fused with the moments of out.
  * Replaces df_dt_fvm (bolt/lib/nonlinear/finite_volume/df_dt_fvm.py:33-339)
  * together with one stage of timestep_fvm (fvm_operator.py:22-33) and the
- * BGK source (bolt/src/nonrelativistic_boltzmann/collision_operator.py:70-128). */
+ * BGK source (bolt/src/nonrelativistic_boltzmann/collision_operator.py:70-128).
+ * In the q1-line layout the q2 extent is marched in segments of about 16
+ * rows; the environment variable BOLTX_QL_SEGMENTS=<count>, read at each
+ * call, fixes the segment count instead (a tuning and test knob: results are
+ * bit-identical for every count). */
 int boltx_fvm_stage(const boltx_grid *grid, const boltx_physics *phys,
                     const boltx_stage *stage, void *stream);
 

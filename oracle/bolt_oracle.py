@@ -478,9 +478,21 @@ class OracleFields:
             self.yee_grid_to_cell_centered_grid("B")
         else:
             raise NotImplementedError("Method not valid/not implemented")
-        # fields.py:493-494: Python aliasing, not copies.
-        self.cell_centered_EM_fields_at_n = self.cell_centered_EM_fields
-        self.cell_centered_EM_fields_at_n_plus_half = self.cell_centered_EM_fields
+        # How the electrodynamic fields follow the RK stages: the reference's
+        # toggle (RK2, the only stepper the reference has), or one leapfrog
+        # per step driven by the stepper (SSP-RK3, timestep_fvm_ssprk3).
+        rk3 = getattr(p, "time_stepper_fvm", "RK2") in ("SSP-RK3", "RK3")
+        self.coupling = ("once-per-step" if rk3 and p.fields_type == "electrodynamic"
+                         else "toggle")
+        self.stage_fields = None
+        if self.coupling == "toggle":
+            # fields.py:493-494: Python aliasing, not copies.
+            self.cell_centered_EM_fields_at_n = self.cell_centered_EM_fields
+            self.cell_centered_EM_fields_at_n_plus_half = self.cell_centered_EM_fields
+        else:
+            # separate arrays; at t = 0, B^{-1/2} = B^{1/2} (fields.py:491)
+            self.cell_centered_EM_fields_at_n = self.cell_centered_EM_fields.clone()
+            self.cell_centered_EM_fields_at_n_plus_half = self.cell_centered_EM_fields.clone()
 
     def _bc(self, x):
         """communicate_fields + periodic apply_bcs_fields: refill the ghosts
@@ -607,9 +619,38 @@ class OracleFields:
         self.cell_centered_EM_fields_at_n_plus_half[:3] = E_at_n_plus_half
         self.time_elapsed += dt
 
+    def evolve_once_per_step(self, J1, J2, J3, dt):
+        """The once-per-step leapfrog of SSP-RK3 (timestep_fvm_ssprk3) with
+        the time levels of fields.py:568-647, kept in separate arrays: before
+        it the cell-centred fields hold (E^n, B^{n+1/2}); after it they hold
+        (E^{n+1}, B^{n+3/2}), ``..._at_n`` holds (E^{n+1}, B^{n+1}) with
+        B^{n+1} = (B^{n+1/2} + B^{n+3/2}) / 2, and ``..._at_n_plus_half``
+        holds (E^{n+1/2}, B^{n+1/2}) with E^{n+1/2} = (E^n + E^{n+1}) / 2."""
+        self.J1 = J1.sum(dim=1, keepdim=True)
+        self.J2 = J2.sum(dim=1, keepdim=True)
+        self.J3 = J3.sum(dim=1, keepdim=True)
+        E_n = self.cell_centered_EM_fields[:3].clone()
+        B_n_plus_half = self.cell_centered_EM_fields[3:].clone()
+        self.fdtd(dt)
+        self.yee_grid_to_cell_centered_grid()
+        E_n_plus_one = self.cell_centered_EM_fields[:3].clone()
+        B_n_plus_3half = self.cell_centered_EM_fields[3:].clone()
+        self.cell_centered_EM_fields_at_n = torch.cat(
+            [E_n_plus_one, 0.5 * (B_n_plus_half + B_n_plus_3half)])
+        self.cell_centered_EM_fields_at_n_plus_half = torch.cat(
+            [0.5 * (E_n + E_n_plus_one), B_n_plus_half])
+        self.time_elapsed += dt
+
     def get_fields(self):
         """fields.py:649-690 (FDTD branch; toggles ``at_n`` on every call when
-        the p-solver is FVM)."""
+        the p-solver is FVM).  With the once-per-step coupling the stepper
+        sets ``stage_fields`` for each stage (outside a step: the fields at
+        n) and nothing toggles."""
+        if self.coupling == "once-per-step":
+            src = self.stage_fields
+            if src is None:
+                src = self.cell_centered_EM_fields_at_n
+            return tuple(src[i:i + 1].clone() for i in range(6))
         if self.params.fields_solver != "fdtd":
             src = self.cell_centered_EM_fields
         else:
@@ -834,6 +875,35 @@ class OracleSolver:
         return fn(f, self.p1_center, self.p2_center, self.p3_center,
                   self.dp3 * self.dp2 * self.dp1)
 
+    def _q_faces(self, f):
+        """df_dt_fvm.py:54-80: reconstruction along q1 / q2, the face states
+        either side of every left / bottom face, the face velocities C_q and
+        the upwind states f_left / f_bot (riemann.py:8-58)."""
+        p = self.params
+        rq = p.reconstruction_method_in_q
+        f_left_plus_eps, f_right_minus_eps = reconstruct(f, 2, rq)
+        f_bot_plus_eps, f_top_minus_eps = reconstruct(f, 3, rq)
+        f_left_minus_eps = af_shift(f_right_minus_eps, 0, 0, 1)
+        f_bot_minus_eps = af_shift(f_top_minus_eps, 0, 0, 0, 1)
+        C_q1 = as4(self._C_q(self.time_elapsed, self.q1_left_center, self.q2_left_center,
+                             self.p1_center, self.p2_center, self.p3_center, p)[0])
+        C_q2 = as4(self._C_q(self.time_elapsed, self.q1_center_bot, self.q2_center_bot,
+                             self.p1_center, self.p2_center, self.p3_center, p)[1])
+        f_left = riemann_upwind_state(f_left_minus_eps, f_left_plus_eps, C_q1)
+        f_bot = riemann_upwind_state(f_bot_minus_eps, f_bot_plus_eps, C_q2)
+        return (f_left_minus_eps, f_left_plus_eps, f_bot_minus_eps, f_bot_plus_eps,
+                C_q1, C_q2, f_left, f_bot)
+
+    def currents(self, f):
+        """J1, J2, J3 of df_dt_fvm.py:214-224: the charge times the bulk
+        momenta of the upwind face states f_left (J1, at (i, j+1/2)) and f_bot
+        (J2, at (i+1/2, j)) and of f itself (J3)."""
+        f_left, f_bot = self._q_faces(f)[6:]
+        J1 = self.charge * self.compute_moments("mom_v1_bulk", f=f_left)
+        J2 = self.charge * self.compute_moments("mom_v2_bulk", f=f_bot)
+        J3 = self.charge * self.compute_moments("mom_v3_bulk", f=f)
+        return J1, J2, J3
+
     def df_dt_fvm(self, f):
         """df_dt_fvm.py:33-339 (the hybrid-model and electrostatic branches are
         not restated)."""
@@ -841,18 +911,9 @@ class OracleSolver:
         df_dt = 0
         f_left = f_bot = None
         if p.solver_method_in_q == "FVM":
-            rq = p.reconstruction_method_in_q
-            f_left_plus_eps, f_right_minus_eps = reconstruct(f, 2, rq)
-            f_bot_plus_eps, f_top_minus_eps = reconstruct(f, 3, rq)
-            f_left_minus_eps = af_shift(f_right_minus_eps, 0, 0, 1)
-            f_bot_minus_eps = af_shift(f_top_minus_eps, 0, 0, 0, 1)
-            C_q1 = as4(self._C_q(self.time_elapsed, self.q1_left_center, self.q2_left_center,
-                                 self.p1_center, self.p2_center, self.p3_center, p)[0])
-            C_q2 = as4(self._C_q(self.time_elapsed, self.q1_center_bot, self.q2_center_bot,
-                                 self.p1_center, self.p2_center, self.p3_center, p)[1])
+            (f_left_minus_eps, f_left_plus_eps, f_bot_minus_eps, f_bot_plus_eps,
+             C_q1, C_q2, f_left, f_bot) = self._q_faces(f)
             rs = getattr(p, "riemann_solver_in_q", "upwind-flux")
-            f_left = riemann_upwind_state(f_left_minus_eps, f_left_plus_eps, C_q1)
-            f_bot = riemann_upwind_state(f_bot_minus_eps, f_bot_plus_eps, C_q2)
             left_flux = face_flux(rs, f_left_minus_eps, f_left_plus_eps, C_q1)
             bot_flux = face_flux(rs, f_bot_minus_eps, f_bot_plus_eps, C_q2)
             right_flux = af_shift(left_flux, 0, 0, -1)
@@ -866,7 +927,11 @@ class OracleSolver:
                                              self.compute_moments, p, False)
         if p.solver_method_in_p == "FVM" and p.fields_enabled:
             fs = self.fields_solver
-            if p.fields_type == "electrodynamic" and fs.at_n is False:
+            # reference coupling (RK2): FDTD inside the RHS whenever at_n is
+            # False; the once-per-step coupling of SSP-RK3 advances the fields
+            # from the stepper instead
+            if p.fields_type == "electrodynamic" and fs.coupling == "toggle" \
+                    and fs.at_n is False:
                 J1 = self.charge * self.compute_moments("mom_v1_bulk", f=f_left)
                 J2 = self.charge * self.compute_moments("mom_v2_bulk", f=f_bot)
                 J3 = self.charge * self.compute_moments("mom_v3_bulk", f=f)
@@ -920,18 +985,40 @@ class OracleSolver:
     def timestep_fvm_ssprk3(self, dt):
         """SSP-RK3 (Shu-Osher); added, the reference only has RK2 midpoint.
         u1 = u + dt L(u); u2 = 3/4 u + 1/4 (u1 + dt L(u1));
-        u^{n+1} = 1/3 u + 2/3 (u2 + dt L(u2))."""
+        u^{n+1} = 1/3 u + 2/3 (u2 + dt L(u2)).
+
+        Electrodynamic fields (the reference defines no RK3 coupling; its
+        per-C_p toggle, fields.py:685-688, would run FDTD on 1, 2, 1, 2, ...
+        of the stages and advance the fields 1.5 dt per step): the stages sit
+        at t^n, t^{n+1} and t^{n+1/2}, so stage 1 sees the fields at n, then
+        one FDTD leapfrog with J^{n+1/2} = (J(u^n) + J(u1)) / 2 (u1 is the
+        Euler estimate of u^{n+1}) takes E^n -> E^{n+1} and B^{n+1/2} ->
+        B^{n+3/2}, stage 2 sees the fields at n+1 and stage 3 those at
+        n+1/2."""
+        fs = self.fields_solver
+        once = fs is not None and fs.coupling == "once-per-step"
         self._communicate_f()
         un = self.f
+        if once:
+            J_n = self.currents(un)
+            fs.stage_fields = fs.cell_centered_EM_fields_at_n
         u1 = un + self.df_dt_fvm(un) * dt
         self.f = u1
         self._communicate_f()
         u1 = self.f
+        if once:
+            J_1 = self.currents(u1)
+            fs.evolve_once_per_step(*(0.5 * (a + b) for a, b in zip(J_n, J_1)), dt)
+            fs.stage_fields = fs.cell_centered_EM_fields_at_n
         u2 = 0.75 * un + 0.25 * (u1 + self.df_dt_fvm(u1) * dt)
         self.f = u2
         self._communicate_f()
         u2 = self.f
+        if once:
+            fs.stage_fields = fs.cell_centered_EM_fields_at_n_plus_half
         self.f = (1 / 3) * un + (2 / 3) * (u2 + self.df_dt_fvm(u2) * dt)
+        if once:
+            fs.stage_fields = None
 
     def check_divergence(self):
         """timestep.py:28-38."""

@@ -2,8 +2,9 @@
 
 The reference ships no golden vectors, so these are oracle outputs on small
 seeded problems: the 1D1V minmod/RK2 case after 1 and 100 steps, one step of a
-small 2D2V problem per reconstruction x stepper, one step of a small 2D3V
-electromagnetic problem, ten steps of a 1D1V electrostatic (FFT Poisson)
+small 2D2V problem per reconstruction x stepper, one and five steps of a small
+2D3V electromagnetic problem under SSP-RK3 (fields advanced once per step) and
+three under RK2 (the reference's field coupling), ten steps of a 1D1V electrostatic (FFT Poisson)
 problem, five steps of a 2D2V box with mirror, dirichlet and mirror+dirichlet
 walls, and five steps of a two-species 2D2V problem.  They pin the oracle against accidental change and
 give the GPU tests fixed expected outputs.  Run from the repository root:
@@ -27,6 +28,9 @@ def golden_cases():
                 lambda rq=rq, st=st: _dt(case_2d2v(seed=1, N_q=8, N_p=6, recon_q=rq,
                                                    recon_p=rq, stepper=st), 1e-3), 1)
     out["2d3v_em"] = (lambda: _dt(case_2d3v(seed=2, N_q=6, N_p=4), 1e-3), 1)
+    out["2d3v_em_rk2_3steps"] = (lambda: _dt(case_2d3v(seed=4, N_q=6, N_p=4, stepper="RK2"),
+                                             1e-3), 3)
+    out["2d3v_em_5steps"] = (lambda: _dt(case_2d3v(seed=5, N_q=6, N_p=4), 1e-3), 5)
     out["1d1v_electrostatic_10steps"] = (lambda: case_es(seed=3), 10)
     out["2d2v_walls_5steps"] = (lambda: case_bc(
         left="mirror", right="dirichlet", bottom="mirror+dirichlet", top="mirror",

@@ -99,7 +99,7 @@ def test_stage_1d1v_singleton_q2():
 def test_stage_fields_p_advection(rp):
     """Fields on, first RK stage (at_n True: no FDTD); C_p1, C_p2, C_p3 read
     the aliased cell-centred fields."""
-    c = case_2d3v(seed=5, N_q=6, N_p=8, recon_p=rp, B0=0.5)
+    c = case_2d3v(seed=5, N_q=6, N_p=8, recon_p=rp, B0=0.5, stepper="RK2")
     o = oracle_solver(c)
     # give the fields some structure so every C_p term matters
     fs = o.fields_solver

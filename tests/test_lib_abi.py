@@ -88,3 +88,19 @@ def test_adjoint_and_ensemble_reject_bad_input():
     t = _lib.EnsTuning(4, 1024, 4, 16384)
     assert lib.boltx_ensemble_step(1, m, C.byref(t), 1, None, None, None, None, 1, None) == -1
     assert b"dt" in lib.boltx_last_error()
+
+
+def test_adjoint_rejects_large_grid_before_launching():
+    """The size limits of the adjoint's launch grid are checked before its
+    first kernel: on a machine without a GPU the call returns the size error
+    (-3), not a launch failure (-2)."""
+    import ctypes as C
+    lib = _lib.load()
+    g = _lib.Grid(256, 8, 256, 256, 128, 1, 1, 1 / 256, 1 / 8, 0.1, 0.1, 0.1, -10.0, -10.0,
+                  -6.0)
+    ph = _lib.Physics(3, 3, 1, 1, 1, 0, 3, 0, 1.0, -1.0, 1.0, 0.01)
+    adj = _lib.Adjoint(C.c_void_p(16), C.c_void_p(24), C.c_void_p(32), None, C.c_void_p(48),
+                       0.0, 1.0, 1.0, None, C.c_void_p(64))
+    assert lib.boltx_adjoint_stage(C.byref(g), C.byref(ph), C.byref(adj), None) == -3
+    assert b"2^31" in lib.boltx_last_error()
+
