@@ -16,13 +16,15 @@ LAYOUT = {"reference": 0, "q1-lines": 1}
 BC = {"periodic": 0, "none": 0, "mirror": 1, "dirichlet": 2, "mirror+dirichlet": 3}
 STEPPER = {"RK2": 0, "SSP-RK3": 1}
 
-EXPORTS = ("boltx_fvm_stage", "boltx_moments", "boltx_workspace_size", "boltx_workspace_init",
+EXPORTS = ("boltx_fvm_stage", "boltx_fvm_stage_noalloc", "boltx_moments",
+           "boltx_workspace_size", "boltx_workspace_init",
            "boltx_currents", "boltx_fdtd_step", "boltx_yee_to_cell_centered",
            "boltx_check_finite", "boltx_adjoint_stage", "boltx_adjoint_workspace_size",
            "boltx_ensemble_layout", "boltx_ensemble_workspace_size", "boltx_ensemble_map",
            "boltx_ensemble_init", "boltx_ensemble_step", "boltx_fill_ghosts",
            "boltx_version", "boltx_last_error")
 MOM_ROWS = 12
+SCRATCH_BYTES = 64 << 20  # BOLTX_SCRATCH_BYTES
 
 
 class Grid(ctypes.Structure):
@@ -91,6 +93,7 @@ def load(path=LIB_PATH):
     vp, i32, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_double
     g, ph, st = ctypes.POINTER(Grid), ctypes.POINTER(Physics), ctypes.POINTER(Stage)
     lib.boltx_fvm_stage.argtypes = [g, ph, st, vp]
+    lib.boltx_fvm_stage_noalloc.argtypes = [g, ph, st, vp]
     lib.boltx_moments.argtypes = [g, ph, vp, vp, vp, vp]
     lib.boltx_workspace_size.argtypes = [g, ph]
     lib.boltx_workspace_init.argtypes = [g, ph, vp, vp]

@@ -829,6 +829,17 @@ extern "C" size_t boltx_workspace_size(const boltx_grid* grid, const boltx_physi
   return qc_workspace_bytes(qc_nslab(grid));
 }
 
+// The sign-grouped p order of the reference layout, written into the first
+// qc_workspace_bytes() of the workspace (every entry, padding included).
+static int init_slab_perm(const boltx_grid* grid, void* workspace, hipStream_t s) {
+  const int nslab = qc_nslab(grid);
+  int* perm = (int*)workspace;
+  hipLaunchKernelGGL(slab_perm_kernel, dim3(1), dim3(64), 0, s, perm, perm + (size_t)nslab * 64,
+                     grid->n_p1, grid->n_p2, grid->n_p3, grid->p1_start, grid->dp1,
+                     grid->p2_start, grid->dp2);
+  return launch_status("slab_perm_kernel");
+}
+
 extern "C" int boltx_workspace_init(const boltx_grid* grid, const boltx_physics* phys,
                                     void* workspace, void* stream) {
   int rc = check_grid(grid);
@@ -836,12 +847,60 @@ extern "C" int boltx_workspace_init(const boltx_grid* grid, const boltx_physics*
   if ((rc = check_physics(phys))) return rc;
   if (grid->layout == BOLTX_LAYOUT_Q1_LINES || !qc_applies(grid)) return BOLTX_OK;
   if (!workspace) return fail(BOLTX_EINVAL, "workspace is NULL");
-  const int nslab = qc_nslab(grid);
-  int* perm = (int*)workspace;
-  hipLaunchKernelGGL(slab_perm_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, perm,
-                     perm + (size_t)nslab * 64, grid->n_p1, grid->n_p2, grid->n_p3,
-                     grid->p1_start, grid->dp1, grid->p2_start, grid->dp2);
-  return launch_status("slab_perm_kernel");
+  return init_slab_perm(grid, workspace, (hipStream_t)stream);
+}
+
+// ---- the stage without a caller workspace ------------------------------------
+// Device memory owned by the library for boltx_fvm_stage_noalloc: one copy per
+// device (each GPU the library runs on holds it until the process ends),
+// shared by every call on that device.  In host code ``scratch`` names a
+// host-side stand-in: kernels get the device address from scratch_address().
+static __device__ __align__(256) unsigned char scratch[BOLTX_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+
+// This device's address of ``scratch``, looked up on the first call with that
+// device current and kept per device.
+static int scratch_address(void** out) {
+  static std::mutex mu;
+  static void* addr[kMaxDevices] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(BOLTX_ELAUNCH, "hipGetDevice failed");
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(BOLTX_EUNSUPPORTED, "device %d is beyond the %d the scratch table holds", dev,
+                kMaxDevices);
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(BOLTX_ELAUNCH, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                  hipGetErrorString(e));
+    addr[dev] = p;
+  }
+  *out = addr[dev];
+  return BOLTX_OK;
+}
+
+extern "C" int boltx_fvm_stage_noalloc(const boltx_grid* grid, const boltx_physics* phys,
+                                       const boltx_stage* st, void* stream) {
+  int rc = check_stage(grid, phys, st);
+  if (rc) return rc;
+  if (st->workspace)
+    return fail(BOLTX_EINVAL, "workspace must be NULL: this entry uses the library's scratch");
+  const size_t need = boltx_workspace_size(grid, phys);
+  if (need > (size_t)BOLTX_SCRATCH_BYTES)
+    return fail(BOLTX_EUNSUPPORTED,
+                "the problem needs %zu bytes of workspace, above the %zu of the scratch array",
+                need, (size_t)BOLTX_SCRATCH_BYTES);
+  hipStream_t s = (hipStream_t)stream;
+  void* ws = nullptr;
+  if (need > 0) {
+    if ((rc = scratch_address(&ws))) return rc;
+    // the previous call may have left another problem's p order here
+    if (grid->layout == BOLTX_LAYOUT_REFERENCE && (rc = init_slab_perm(grid, ws, s))) return rc;
+  }
+  return fvm_stage_impl(grid, phys, st, ws, s);
 }
 
 template <typename T>

@@ -124,6 +124,20 @@ typedef struct boltx_stage {
 int boltx_fvm_stage(const boltx_grid *grid, const boltx_physics *phys,
                     const boltx_stage *stage, void *stream);
 
+/* boltx_fvm_stage with no workspace argument and no allocation.  It has no
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  stage->workspace must be NULL; the workspace lives
+ * in a static device array of BOLTX_SCRATCH_BYTES that the library holds on
+ * every GPU it runs on until the process ends.  A problem whose
+ * boltx_workspace_size() exceeds it is rejected with BOLTX_EUNSUPPORTED.
+ * Every call on one GPU shares that GPU's array: calls whose device work
+ * overlaps (other streams, other host threads) race, so order them on one
+ * stream.  Outputs equal those of boltx_fvm_stage bit for bit. */
+#define BOLTX_SCRATCH_BYTES (64u << 20)
+int boltx_fvm_stage_noalloc(const boltx_grid *grid, const boltx_physics *phys,
+                            const boltx_stage *stage, void *stream);
+
 /* The moment block of f.  Replaces compute_moments
  * (bolt/lib/nonlinear/compute_moments.py:7-59) for density, mom_v1_bulk,
  * mom_v2_bulk, mom_v3_bulk and energy (moments.py:5-22), plus the BGK

@@ -104,3 +104,24 @@ def test_adjoint_rejects_large_grid_before_launching():
     assert lib.boltx_adjoint_stage(C.byref(g), C.byref(ph), C.byref(adj), None) == -3
     assert b"2^31" in lib.boltx_last_error()
 
+
+
+def test_noalloc_stage_rejects_before_device_work():
+    """boltx_fvm_stage_noalloc refuses a caller workspace and a problem whose
+    workspace exceeds its scratch array, before touching the device (on a
+    machine without a GPU the calls return these errors, not a launch
+    failure)."""
+    import ctypes as C
+    lib = _lib.load()
+    g = _lib.Grid(256, 4096, 48, 48, 1, 1, 1, 1 / 256, 1 / 4096, 20 / 48, 20 / 48, 1.0,
+                  -10.0, -10.0, -0.5)
+    ph = _lib.Physics(3, 3, 1, 1, 1, 0, 2, 0, 1.0, -1.0, 1.0, 0.01)
+    assert lib.boltx_workspace_size(C.byref(g), C.byref(ph)) > _lib.SCRATCH_BYTES
+    st = _lib.Stage()
+    st.u, st.w, st.out, st.mom_in = C.c_void_p(8), C.c_void_p(8), C.c_void_p(16), C.c_void_p(24)
+    st.a, st.b, st.cdt = 0.0, 1.0, 1e-3
+    assert lib.boltx_fvm_stage_noalloc(C.byref(g), C.byref(ph), C.byref(st), None) == -3
+    assert b"scratch" in lib.boltx_last_error()
+    st.workspace = C.c_void_p(32)
+    assert lib.boltx_fvm_stage_noalloc(C.byref(g), C.byref(ph), C.byref(st), None) == -1
+    assert b"NULL" in lib.boltx_last_error()
