@@ -8,12 +8,17 @@ SSP-RK3, dt = 5e-4, plain FP64 on the GPU.  A step is one full SSP-RK3 step
 beside it and its rel-L2 against the FP64 step is reported.
 
 Prints one JSON line.  ``value`` is FP64 phase-space cell-updates/s with f
-resident in HBM.  ``roofline`` is for the dominant kernel (the fused stage),
-with the device-copy bandwidth measured on the same GPU beside the nominal
-peak (``copy_gbs``, ``frac_of_copy``):
-algorithmic bytes (each stage charged one read per f-sized input and one write,
-8 passes per SSP-RK3 step = 64 B/cell in FP64, 32 B/cell in FP32-bulk) divided
-by its measured launch time (HIP events on the launch stream).
+resident in HBM: after at least 1 s of warm-up steps (``--warmup`` steps or
+more), exactly ``--steps`` steps are timed in five repeats of near-equal length
+and the median repeat gives ``ms_per_step``.  ``roofline`` is for the dominant
+kernel (the fused stage), with the device-copy bandwidth measured on the same
+GPU beside the nominal peak (``copy_gbs``, ``frac_of_copy``): algorithmic bytes
+(each stage charged one read per f-sized input and one write, 8 passes per
+SSP-RK3 step = 64 B/cell in FP64, 32 B/cell in FP32-bulk) divided by its
+measured launch time (HIP events on the launch stream); ``step_achieved`` /
+``step_frac`` divide the same bytes by the whole step's time (every launch of
+the step, moment passes included).  ``traffic`` is the counter-measured HBM
+bytes per launch (``--probe`` / ``--traffic``).
 """
 import argparse
 import json
@@ -29,22 +34,57 @@ METRIC = ("phase-space cell-updates/s, FP32-bulk vs FP64 at rel-L2≤1e-5; "
           "% HBM peak")
 
 
-def run_gpu(precision, steps, warmup):
+def warm_up(step, warmup, min_seconds=1.0):
+    """``warmup`` untimed steps, then more until ``min_seconds`` of stepping
+    have passed (the clocks ramp up from idle over the first ~0.5 s, whatever
+    step count the caller asked for).  Returns the steps run."""
     import torch
+    t0 = time.perf_counter()
+    n = 0
+    while n < warmup or time.perf_counter() - t0 < min_seconds:
+        step()
+        n += 1
+        if n >= warmup:
+            torch.cuda.synchronize()
+    torch.cuda.synchronize()
+    return n
+
+
+def timed_repeats(step, steps, repeats=5):
+    """Exactly ``steps`` steps, split into ``min(repeats, steps)`` timed
+    repeats of near-equal length, each bracketed by device synchronisation.
+    Returns the ms per step of every repeat and the total wall seconds."""
+    import torch
+    r = max(1, min(repeats, steps))
+    sizes = [steps // r + (1 if k < steps % r else 0) for k in range(r)]
+    ms, total = [], 0.0
+    for n in sizes:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            step()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        total += dt
+        ms.append(dt / n * 1e3)
+    return ms, total
+
+
+def _median(x):
+    y = sorted(x)
+    n = len(y)
+    return y[n // 2] if n % 2 else 0.5 * (y[n // 2 - 1] + y[n // 2])
+
+
+def run_gpu(precision, steps, warmup):
     from tests.cases import case_2d2v, hip_solver
     c = case_2d2v(seed=0, precision=precision)
     nls = hip_solver(c)
-    for _ in range(warmup):
-        nls._advance(c.dt)
+    warm = warm_up(lambda: nls._advance(c.dt), warmup)
     nls.check_divergence()
     events = []
     nls._stage_events = events
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        nls._advance(c.dt)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    ms, wall = timed_repeats(lambda: nls._advance(c.dt), steps)
     nls.check_divergence()
     nls._stage_events = None
     stage_ms = [e0.elapsed_time(e1) for e0, e1 in events]
@@ -52,28 +92,84 @@ def run_gpu(precision, steps, warmup):
     bytes_per_cell_step = 8 * (8 if precision == 'fp64' else 4)
     kernel_s = sum(stage_ms) / 1e3
     achieved = bytes_per_cell_step * cells * steps / kernel_s / 1e9
+    ms_step = _median(ms)
+    step_gbs = bytes_per_cell_step * cells / (ms_step / 1e3) / 1e9
     traffic = _traffic(precision)
     return nls, {
         "cells": cells,
-        "ms_per_step": wall / steps * 1e3,
-        "value": cells * steps / wall,
+        "ms_per_step": ms_step,
+        "ms_per_step_repeats": ms,
+        "warmup_steps_run": warm,
+        "timed_wall_s": wall,
+        "value": cells / (ms_step / 1e3),
         "stage_launch_ms_avg": sum(stage_ms) / len(stage_ms),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "step_achieved": step_gbs, "step_frac": step_gbs / HBM_PEAK_GBS},
         "kernel": "qline_kernel (stage stencil; SSP-RK3 stages average 8/3 f-passes)",
     }
 
 
-def _traffic(precision):
-    """HBM bytes per stage-kernel launch measured with rocprofv3 --pmc
-    (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration); the
-    measured figures are kept in profiles/traffic.json."""
+def _traffic(key):
+    """HBM bytes per launch of the workload's dominant kernel measured with
+    rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel
+    calibration), as ``bench.py --traffic`` wrote them into
+    profiles/traffic.json; None if not measured."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as fh:
-            return json.load(fh).get(precision, {}).get("bytes_per_launch")
+            return json.load(fh).get(key, {}).get("bytes_per_launch")
     except (OSError, ValueError):
         return None
+
+
+def _pmc_rows(path, counter):
+    """(kernel name, value) per dispatch of ``counter`` from a rocprofv3
+    counter-collection CSV, in dispatch order."""
+    import csv
+    rows = []
+    with open(path, newline="") as fh:
+        for r in csv.DictReader(fh):
+            if r.get("Counter_Name") == counter:
+                rows.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"])))
+    rows.sort()
+    return [(k, v) for _, k, v in rows]
+
+
+def traffic_from_pmc(key, fetch_csv, write_csv, kernel, algorithmic):
+    """HBM bytes per launch of ``kernel`` from one FETCH_SIZE run and one
+    WRITE_SIZE run of ``bench.py --probe`` (KiB per dispatch).  Each counter
+    is scaled by what it reads for the probe's first 256 MiB device copy
+    (read 256 MiB, write 256 MiB) in the same run.  ``algorithmic`` are the
+    algorithmic bytes of each launch, in order.  Updates
+    profiles/traffic.json under ``key`` and returns the entry."""
+    mib256 = 256 * 1024.0  # KiB
+    out = {}
+    for name, path in (("fetch", fetch_csv), ("write", write_csv)):
+        rows = _pmc_rows(path, name.upper() + "_SIZE")
+        cal = [v for k, v in rows if "copy" in k.lower()]
+        launches = [v for k, v in rows if kernel in k]
+        if not cal or not launches:
+            raise SystemExit("no copy or %s dispatches in %s" % (kernel, path))
+        out[name + "_scale"] = mib256 / cal[0]
+        out[name + "_kib_per_launch"] = launches
+    n = min(len(out["fetch_kib_per_launch"]), len(out["write_kib_per_launch"]), len(algorithmic))
+    per = [1024.0 * (out["fetch_kib_per_launch"][i] * out["fetch_scale"] +
+                     out["write_kib_per_launch"][i] * out["write_scale"]) for i in range(n)]
+    entry = dict(out, kernel=kernel, per_launch_bytes=per,
+                 algorithmic_per_launch_bytes=list(algorithmic[:n]),
+                 bytes_per_launch=sum(per) / n,
+                 ratio_to_algorithmic=sum(per) / sum(algorithmic[:n]))
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            table = json.load(fh)
+    except (OSError, ValueError):
+        table = {}
+    table[key] = entry
+    with open(path, "w") as fh:
+        json.dump(table, fh, indent=1)
+    return entry
 
 
 def copy_bandwidth(mib=1024, reps=10):
@@ -134,46 +230,63 @@ def run_adjoint(steps=100, every=10):
 def run_em(steps=10, warmup=2):
     """2D3V electromagnetic step: 64^2 q x 24^3 p on [-8, 8]^3, electrons
     (q/m = -1) in B = (0, 0, 0.5), PPM in q, WENO5 in p, LLF, BGK tau = 0.1,
-    SSP-RK3 dt = 2e-4, Yee FDTD fields; FP32-bulk and plain FP64, with the
-    FP32-bulk error against the FP64 run after the same steps."""
+    SSP-RK3 dt = 2e-4 with the fields advanced once per step (Yee FDTD);
+    FP32-bulk and plain FP64, with the FP32-bulk error against the FP64 run
+    after the same steps.  The roofline is that of the stage kernel
+    (qcell_kernel with p-advection) at the algorithmic bytes of an SSP-RK3
+    stage (8/3 f-passes), beside the whole step's algorithmic rate."""
     import torch
     from tests.cases import case_2d3v, hip_solver
     out = {}
-    f = {}
     for prec in ("fp64", "fp32-bulk"):
         c = case_2d3v(seed=0, precision=prec)
         s = hip_solver(c)
-        for _ in range(warmup):
-            s._advance(c.dt)
+        warm = warm_up(lambda: s._advance(c.dt), warmup)
         events = []
         s._stage_events = events
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            s._advance(c.dt)
-        torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / steps
+        ms, _ = timed_repeats(lambda: s._advance(c.dt), steps)
         s._stage_events = None
         s.check_divergence()
-        # the stage kernel (qcell_kernel with p-advection) against the HBM
-        # roofline at the algorithmic bytes of an SSP-RK3 stage (8/3 f-passes)
         stage_s = sum(e0.elapsed_time(e1) for e0, e1 in events) / len(events) / 1e3
         esz = 8 if prec == "fp64" else 4
-        achieved = 8 / 3 * s._Q * s._P * esz / stage_s / 1e9
-        out[prec] = {"ms_per_step": dt * 1e3, "value": s._Q * s._P / dt,
+        cells = s._Q * s._P
+        achieved = 8 / 3 * cells * esz / stage_s / 1e9
+        ms_step = _median(ms)
+        step_gbs = 8 * esz * cells / (ms_step / 1e3) / 1e9
+        out[prec] = {"ms_per_step": ms_step, "ms_per_step_repeats": ms,
+                     "warmup_steps_run": warm, "value": cells / (ms_step / 1e3),
                      "stage_launch_ms_avg": stage_s * 1e3,
                      "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                                   "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                                  "traffic": None}}
+                                  "traffic": _traffic("em_" + prec),
+                                  "step_achieved": step_gbs,
+                                  "step_frac": step_gbs / HBM_PEAK_GBS},
+                     "kernel": "qcell_kernel (stage with p-advection, 8/3 f-passes per launch)"}
+        del s
+    n_err = 20
+    return {"workload": "2D3V EM 64^2 q x 24^3 p, PPM q / WENO5 p, LLF, BGK tau=0.1, "
+                        "SSP-RK3 dt=2e-4, FDTD fields once per step",
+            "cells": 64 * 64 * 24 ** 3, "fp32_bulk": out["fp32-bulk"], "fp64": out["fp64"],
+            "fp32_bulk_rel_l2_vs_fp64_after_%d_steps" % n_err: fp32_bulk_error(case_2d3v, n_err),
+            "speedup_fp32_bulk_vs_fp64": out["fp32-bulk"]["value"] / out["fp64"]["value"]}
+
+
+def fp32_bulk_error(case_fn, n_steps):
+    """rel-L2 of f between the FP32-bulk and the plain FP64 GPU step after
+    ``n_steps`` steps from the same initial state (untimed)."""
+    import torch
+    from tests.cases import hip_solver
+    f = {}
+    for prec in ("fp64", "fp32-bulk"):
+        c = case_fn(seed=0, precision=prec)
+        s = hip_solver(c)
+        for _ in range(n_steps):
+            s._advance(c.dt)
+        s.check_divergence()
         f[prec] = s._f.double()
         del s
-    rel = (torch.linalg.vector_norm(f["fp32-bulk"] - f["fp64"]) /
-           torch.linalg.vector_norm(f["fp64"])).item()
-    return {"workload": "2D3V EM 64^2 q x 24^3 p, PPM q / WENO5 p, LLF, BGK tau=0.1, "
-                        "SSP-RK3 dt=2e-4, FDTD fields",
-            "cells": 64 * 64 * 24 ** 3, "fp32_bulk": out["fp32-bulk"], "fp64": out["fp64"],
-            "fp32_bulk_rel_l2_vs_fp64_after_%d_steps" % (steps + warmup): rel,
-            "speedup_fp32_bulk_vs_fp64": out["fp32-bulk"]["value"] / out["fp64"]["value"]}
+    return (torch.linalg.vector_norm(f["fp32-bulk"] - f["fp64"]) /
+            torch.linalg.vector_norm(f["fp64"])).item()
 
 
 def run_ensemble(steps=200, sample_steps=10):
@@ -227,11 +340,16 @@ def run_ensemble(steps=200, sample_steps=10):
         worst = max(worst, float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)))
     rate = cells * steps / t_ens
     seq = cells * sample_steps / t_seq
+    # whole-step algorithmic bytes (SSP-RK3, FP32-bulk: 8 passes of 4 B)
+    step_gbs = 32 * cells * steps / t_ens / 1e9
     return {"workload": "96 x 1D2V ensemble (N_q1 in {32..256}, N_p in {24..64}^2, own dt and "
                         "tau), WENO5 + LLF + BGK, SSP-RK3, FP32-bulk, %d steps" % steps,
             "members": len(dts), "cells": cells, "value": rate,
             "unit": "phase-space cell-updates/s", "ms_per_step": t_ens / steps * 1e3,
             "ms_per_step_without_graph": t_eager / steps * 1e3,
+            "roofline": {"bound": "hbm", "level": "step (all launches of a step)",
+                         "achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": step_gbs / HBM_PEAK_GBS, "traffic": None},
             "tuning": {"waves_per_group": ens.tuning[0], "cells_per_task": ens.tuning[1],
                        "groups_per_cu": ens.tuning[2], "chunk_cells": ens.tuning[3]},
             "autotune_s": t_tune,
@@ -252,11 +370,12 @@ def _cpu_model():
     return "unknown"
 
 
-def cpu_baseline():
+def cpu_baseline(steps=3):
     """The FP64 oracle (torch CPU eager, the reference's whole-array op
-    sequence) on bounded samples of the same problem: 64^2 q-cells x 48^2
-    p-cells with every host thread this process may use, and 32^2 x 24^2 on
-    one thread; one SSP-RK3 step each."""
+    sequence) on the full headline grid, 128^2 q-cells x 48^2 p-cells, over
+    ``steps`` SSP-RK3 steps with every host thread this process may use; and
+    on one thread over one step of 32^2 x 48^2 (the same p-grid, a quarter
+    of the q-rows per axis)."""
     import torch
     from tests.cases import case_2d2v, oracle_solver
     try:
@@ -264,59 +383,86 @@ def cpu_baseline():
     except AttributeError:
         n = os.cpu_count() or 1
 
-    def rate(threads, nq, npp):
+    def rate(threads, nq, npp, k):
         torch.set_num_threads(threads)
         c = case_2d2v(seed=0, N_q=nq, N_p=npp)
         o = oracle_solver(c)
         t0 = time.perf_counter()
-        o.strang_timestep(c.dt)
-        return nq * nq * npp * npp / (time.perf_counter() - t0)
+        for _ in range(k):
+            o.strang_timestep(c.dt)
+        return nq * nq * npp * npp * k / (time.perf_counter() - t0)
 
-    v = rate(n, 64, 48)
-    v1 = rate(1, 32, 24)
+    v = rate(n, 128, 48, steps)
+    v1 = rate(1, 32, 48, 1)
     torch.set_num_threads(n)
     return {"value": v, "unit": "phase-space cell-updates/s", "cores": n,
             "kind": "port",
-            "sample": "FP64 oracle (torch CPU), 2D2V WENO5/LLF/BGK SSP-RK3 at "
-                      "64^2 x 48^2 cells, 1 step, %d threads" % n,
+            "sample": "FP64 oracle (torch CPU), 2D2V WENO5/LLF/BGK SSP-RK3 on the full "
+                      "128^2 x 48^2 grid, %d steps, %d threads" % (steps, n),
             "cpu_model": _cpu_model(),
-            "one_thread": {"value": v1, "sample": "32^2 x 24^2 cells, 1 step, 1 thread"}}
+            "one_thread": {"value": v1, "sample": "32^2 x 48^2 cells, 1 step, 1 thread"},
+            "thread_speedup": v / v1}
 
 
-def probe(precision):
-    """Launch sequence for counter runs: three copies of a known size, then one
-    SSP-RK3 step (three stage kernels and their moment passes)."""
+def probe(which):
+    """Launch sequence for counter runs: three copies of a known size (256 MiB
+    read and written: the counter calibration), then one SSP-RK3 step of the
+    headline workload (``fp64``, ``fp32-bulk``: three qline_kernel stages
+    and their moment passes) or of the 2D3V electromagnetic one (``em-fp64``,
+    ``em-fp32-bulk``: three qcell_kernel stages)."""
     import torch
-    from tests.cases import case_2d2v, hip_solver
+    from tests.cases import case_2d2v, case_2d3v, hip_solver
     x = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")  # 256 MiB
     y = torch.empty_like(x)
     for _ in range(3):
         y.copy_(x)
     torch.cuda.synchronize()
-    c = case_2d2v(seed=0, precision=precision)
+    if which.startswith("em-"):
+        c = case_2d3v(seed=0, precision=which[3:])
+    else:
+        c = case_2d2v(seed=0, precision=which)
     nls = hip_solver(c)
     nls._advance(c.dt)
     torch.cuda.synchronize()
     print("probe done", flush=True)
 
 
+def probe_algorithmic(which):
+    """Algorithmic bytes of the probe's three stage launches: stage 1 reads
+    u and writes out, stages 2 and 3 also read un."""
+    cells = 64 * 64 * 24 ** 3 if which.startswith("em-") else 128 * 128 * 48 * 48
+    esz = 4 if which.endswith("fp32-bulk") else 8
+    return [2 * cells * esz, 3 * cells * esz, 3 * cells * esz]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
-    # the stage kernel reaches its steady rate after about 30 steps (clocks
-    # ramp up from idle): warm up past that
+    # at least 1 s of warm-up steps is run whatever this says (clocks ramp up
+    # from idle)
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the 2D3V electromagnetic and adjoint measurements")
-    ap.add_argument("--probe", choices=["fp64", "fp32-bulk"],
+    ap.add_argument("--probe", choices=["fp64", "fp32-bulk", "em-fp64", "em-fp32-bulk"],
                     help="traffic probe for rocprofv3 --pmc: a 256 MiB device copy "
                          "(the counter calibration) and then one step of the workload")
+    ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="with --probe-of: turn the counter CSVs of a FETCH_SIZE and a "
+                         "WRITE_SIZE run of that probe into profiles/traffic.json")
+    ap.add_argument("--probe-of", choices=["fp64", "fp32-bulk", "em-fp64", "em-fp32-bulk"])
     ap.add_argument("--only", choices=["em_2d3v", "adjoint", "ensemble"],
                     help="run one of the other workloads alone (for profiling) and print "
                          "its JSON object")
     args = ap.parse_args()
+    if args.traffic:
+        w = args.probe_of
+        kernel = "qcell_kernel" if w.startswith("em-") else "qline_kernel"
+        key = "em_" + w[3:] if w.startswith("em-") else w
+        e = traffic_from_pmc(key, args.traffic[0], args.traffic[1], kernel, probe_algorithmic(w))
+        print(json.dumps({key: e}, indent=1), flush=True)
+        return
     if args.probe:
         probe(args.probe)
         return
@@ -328,15 +474,15 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the step shards over q with a halo exchange per stage)", file=sys.stderr)
         sys.exit(2)
-    import torch
+    from tests.cases import case_2d2v
     copy_gbs = copy_bandwidth()
     nls64, r64 = run_gpu('fp64', args.steps, args.warmup)
+    del nls64
     nls32, r32 = run_gpu('fp32-bulk', args.steps, args.warmup)
+    del nls32
     # accuracy of the FP32-bulk step against the FP64 step, same inputs, same steps
-    f64 = nls64._f
-    f32 = nls32._f.double()
-    rel_l2 = (torch.linalg.vector_norm(f32 - f64) / torch.linalg.vector_norm(f64)).item()
-    del nls64, nls32
+    n_err = 100
+    rel_l2 = fp32_bulk_error(case_2d2v, n_err)
     out = {
         "metric": METRIC,
         "value": r64["value"],
@@ -345,6 +491,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": r64["ms_per_step"],
+        "ms_per_step_repeats": r64["ms_per_step_repeats"],
+        "warmup_steps_run": r64["warmup_steps_run"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -356,10 +504,11 @@ def main():
         "roofline": dict(r64["roofline"], copy_gbs=copy_gbs,
                          frac_of_copy=r64["roofline"]["achieved"] / copy_gbs),
         "fp32_bulk": {"value": r32["value"], "ms_per_step": r32["ms_per_step"],
+                      "ms_per_step_repeats": r32["ms_per_step_repeats"],
                       "dtype": "f32 bulk, f64 moments/temperature",
                       "roofline": dict(r32["roofline"], copy_gbs=copy_gbs,
                                        frac_of_copy=r32["roofline"]["achieved"] / copy_gbs),
-                      "rel_l2_vs_fp64_after_run": rel_l2,
+                      "rel_l2_vs_fp64_after_%d_steps" % n_err: rel_l2,
                       "speedup_vs_fp64": r32["value"] / r64["value"]},
     }
     if not args.no_extras:

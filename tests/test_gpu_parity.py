@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 import torch
 
+from bolt_amd import _lib
 from tests.cases import case_2d2v, case_2d3v, case_c1, hip_solver, oracle_solver
 
 pytestmark = pytest.mark.gpu
@@ -92,6 +93,7 @@ def test_2d2v_weno5_fine_q(gpu):
     o, nls = oracle_solver(c), hip_solver(c)
     o.dt = nls.dt = c.dt
     o._communicate_f()
+    nls.f = o.f.to(nls.device)  # the same input bits
     ref = o.interior(o.f + o.df_dt_fvm(o.f) * c.dt).numpy()
     out = torch.empty_like(nls._f)
     m = torch.empty_like(nls._mom)
@@ -109,6 +111,49 @@ def test_2d2v_weno5_fine_q(gpu):
     print(f"one step at 128^2: rel Linf {linf:.3e}; oracle 1-ulp sensitivity {sens:.3e}",
           flush=True)
     assert linf <= max(1e-12, 4 * sens)
+
+
+def test_c2_full_size_rhs_vs_oracle(gpu):
+    """The headline workload at full size (128^2 q x 48^2 p, WENO5 + LLF +
+    BGK): one RHS evaluation u + dt L(u) from the same state agrees with the
+    oracle to 1e-12 relative L-inf in both storage layouts, and the moment
+    block of the result (density, momenta, energy) to 1e-12.  The momenta of
+    this near-symmetric state cancel to ~1e-5 of their terms, so their error
+    is measured against the terms' size, max(density) * max|p|, not against
+    their own maximum (which would measure the summation order)."""
+    c = case_2d2v(seed=2, N_q=128, N_p=48, riemann="lax-friedrichs")
+    o = oracle_solver(c)
+    o.dt = c.dt
+    o._communicate_f()
+    ref_full = o.f + o.df_dt_fvm(o.f) * c.dt
+    ref = o.interior(ref_full).numpy()
+    names = ("density", "mom_v1_bulk", "mom_v2_bulk", "energy")
+    ref_m = [o.interior(o.compute_moments(n, f=ref_full)).numpy() for n in names]
+    del ref_full
+    for layout in ("q1-lines", "reference"):
+        c.params.memory_layout = layout
+        nls = hip_solver(c)
+        assert nls._layout == _lib.LAYOUT[layout]
+        # the same input bits: the expanded WENO5 indicator turns a 1-ulp
+        # difference of the initial state (device vs host exp) into ~1e-10
+        nls.f = o.f.to(nls.device)
+        out = torch.empty_like(nls._f)
+        m = torch.empty_like(nls._mom)
+        nls._stage(nls._f, nls._f, None, out, 0.0, 1.0, c.dt, nls._mom, m)
+        g1, g2 = nls.N_g1, nls.N_g2
+        sl = (slice(None), slice(None), slice(g1, g1 + nls.N_q1), slice(g2, g2 + nls.N_q2))
+        got = nls._to_reference(out)[sl].cpu().numpy()
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        print(f"C2 full size, one RHS, {layout}: rel Linf {err:.3e}", flush=True)
+        assert err <= 1e-12
+        # rows 0, 1, 2, 4 of the stage's moment block: density, mom_v1_bulk,
+        # mom_v2_bulk, energy
+        for row, n, want in zip((0, 1, 2, 4), names, ref_m):
+            have = nls._q_padded(m[row]).cpu().numpy()[sl]
+            scale = np.abs(ref_m[0]).max() * 10.0 if n.startswith("mom") else np.abs(want).max()
+            e = np.abs(have - want).max() / scale
+            assert e <= 1e-12, (layout, n, e)
+        del nls, out
 
 
 def test_fp32_bulk_error_vs_fp64_oracle(gpu):
