@@ -16,7 +16,29 @@ point; it is written here too and ignored on load.
 
 Only files whose scalars are float64 and whose integers are 32-bit (PETSc's
 defaults, which the reference builds with) are produced and accepted.
+
+Restart state.  The reference's files hold f and the Yee-grid fields only; a
+run restarted from them loses the time, the field time levels the staggered
+update keeps (fields.py:568-647) and, here, the exact bits of the moment
+blocks the solver carries from stage to stage.  Beside each ``name.bin`` the
+dumps write, in the same Vec format where the data are arrays:
+
+  name.state.json          time_elapsed; for fields, the coupling, at_n and
+                           the fields' own time_elapsed; the SHA-256 of
+                           name.bin, so a companion is only used with the
+                           file it was written with;
+  name_moment_blocks.bin   (f dumps) every species' moment block, dof =
+                           12 * S per q-cell;
+  name_cell_centered.bin   (field dumps) the cell-centred fields and their
+                           two time levels (at n, at n+1/2), dof = 18.
+
+The loads restore this state when the companions are present and match, so
+the restarted run continues bit for bit; without them (files the reference
+wrote, or a replaced name.bin) they do what the reference's loads do and
+recompute the moments (and the cell-centred fields) from the loaded arrays.
 """
+import hashlib
+import json
 import os
 import struct
 
@@ -70,9 +92,39 @@ def moment_names(solver):
     return names
 
 
+def _sha256(path):
+    h = hashlib.sha256()
+    with open(path, 'rb') as fh:
+        for chunk in iter(lambda: fh.read(1 << 24), b''):
+            h.update(chunk)
+    return h.hexdigest()
+
+
+def _write_state(file_name, state):
+    state = dict(state, format='bolt_amd restart state 1',
+                 bin_sha256=_sha256(file_name + '.bin'))
+    with open(file_name + '.state.json', 'w') as fh:
+        json.dump(state, fh, indent=1)
+
+
+def _read_state(file_name):
+    """The companion state of file_name.bin, or None when there is none or
+    it belongs to another file."""
+    try:
+        with open(file_name + '.state.json') as fh:
+            state = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if state.get('format') != 'bolt_amd restart state 1' or \
+            state.get('bin_sha256') != _sha256(file_name + '.bin'):
+        return None
+    return state
+
+
 def dump_distribution_function(solver, file_name):
     """dump.py:121-168: f's interior in natural order (dof = p + P s fastest,
-    then q1, then q2), to file_name + '.bin'."""
+    then q1, then q2), to file_name + '.bin'; plus the restart state (time,
+    moment blocks)."""
     s = solver
     parts = []
     for sp in s._spc:
@@ -80,21 +132,39 @@ def dump_distribution_function(solver, file_name):
         parts.append(f)
     f = torch.stack(parts, dim=2)  # [N_q2][N_q1][S][P]
     write_vec(file_name + '.bin', _natural(f), s._P * s.N_species)
+    mom = torch.stack([sp.mom for sp in s._spc], dim=0)  # [S][12][Q]
+    write_vec(file_name + '_moment_blocks.bin', _natural(mom.reshape(-1, s._Q).t()),
+              mom.shape[0] * mom.shape[1])
+    _write_state(file_name, {'time_elapsed': float(s.time_elapsed),
+                             'species': s.N_species, 'moment_rows': int(mom.shape[1])})
 
 
 def load_distribution_function(solver, file_name):
     """load.py:10-43: f from file_name + '.bin' (ghost zones refilled by the
-    boundary conditions before the next RHS)."""
+    boundary conditions before the next RHS).  With the companion state of
+    that file, time_elapsed and the moment blocks are restored as dumped;
+    otherwise the moments are recomputed from f."""
     s = solver
     S = s.N_species
     v = read_vec(file_name + '.bin', s._Q * s._P * S)
     x = torch.from_numpy(v).to(s.device).reshape(s.N_q2, s.N_q1, S, s._P)
+    state = _read_state(file_name)
+    mom = None
+    if state is not None and os.path.exists(file_name + '_moment_blocks.bin'):
+        rows = int(state.get('moment_rows', 0))
+        m = read_vec(file_name + '_moment_blocks.bin', s._Q * S * rows)
+        mom = torch.from_numpy(m).to(s.device).reshape(s._Q, S, rows).permute(1, 2, 0)
     for k in s._species_loop():
         y = x[:, :, k]
         if s._layout != 0:
             y = y.permute(0, 2, 1)
         s._f = s._store(y.to(s._dtype))
-        s._update_moments()
+        if mom is not None and s._mom.shape == mom[k].shape:
+            s._mom.copy_(mom[k])
+        else:
+            s._update_moments()
+    if state is not None:
+        s.time_elapsed = state['time_elapsed']
 
 
 def dump_moments(solver, file_name):
@@ -110,21 +180,48 @@ def dump_moments(solver, file_name):
 
 
 def dump_EM_fields(solver, file_name):
-    """dump.py:170-233: the Yee-grid fields E1, E2, E3, B1, B2, B3."""
+    """dump.py:170-233: the Yee-grid fields E1, E2, E3, B1, B2, B3; plus the
+    restart state (the cell-centred fields at their time levels, at_n, the
+    fields' time)."""
     fs = solver.fields_solver
     if fs is None:
         raise AttributeError('fields are not enabled: there is no fields solver to dump')
     write_vec(file_name + '.bin', _natural(fs._yee.t()), 6)
+    cc = torch.cat([fs._cc, fs._cc_at_n, fs._cc_at_n_plus_half])  # [18][Q]
+    write_vec(file_name + '_cell_centered.bin', _natural(cc.t()), 18)
+    _write_state(file_name, {'time_elapsed': float(solver.time_elapsed),
+                             'fields_time_elapsed': float(fs.time_elapsed),
+                             'at_n': bool(fs.at_n), 'coupling': fs.coupling,
+                             'time_levels_aliased': fs._cc_at_n is fs._cc})
 
 
 def load_EM_fields(solver, file_name):
-    """load.py:45-79: the Yee-grid fields, then their cell-centred values."""
+    """load.py:45-79: the Yee-grid fields, then their cell-centred values.
+    With the companion state of that file the cell-centred time levels, at_n
+    and the fields' time are restored as dumped (a restart then continues
+    the staggered update exactly)."""
     fs = solver.fields_solver
     if fs is None:
         raise AttributeError('fields are not enabled: there is no fields solver to load into')
     v = read_vec(file_name + '.bin', 6 * solver._Q)
     fs._yee.copy_(torch.from_numpy(v).to(solver.device).reshape(solver._Q, 6).t())
-    fs._yee_to_cc(fs._cc)
+    state = _read_state(file_name)
+    if state is None or state.get('coupling') != fs.coupling or \
+            not os.path.exists(file_name + '_cell_centered.bin'):
+        fs._yee_to_cc(fs._cc)
+        return
+    c = read_vec(file_name + '_cell_centered.bin', 18 * solver._Q)
+    cc = torch.from_numpy(c).to(solver.device).reshape(solver._Q, 18).t()
+    fs._cc.copy_(cc[0:6])
+    if state.get('time_levels_aliased'):
+        # fields.py:493-494: the three names refer to one array
+        fs._cc_at_n = fs._cc_at_n_plus_half = fs._cc
+    else:
+        fs._cc_at_n = cc[6:12].contiguous()
+        fs._cc_at_n_plus_half = cc[12:18].contiguous()
+    fs.at_n = state['at_n']
+    fs.time_elapsed = state['fields_time_elapsed']
+    solver.time_elapsed = state['time_elapsed']
 
 
 def dump_aux_arrays(solver, arrays, name, file_name):

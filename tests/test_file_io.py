@@ -51,3 +51,20 @@ def test_natural_order_matches_af_flat_of_the_interior(tmp_path):
     path = str(tmp_path / "f.bin")
     file_io.write_vec(path, file_io._natural(storage), P)
     assert np.array_equal(file_io.read_vec(path), want)
+
+
+def test_restart_state_belongs_to_its_file(tmp_path):
+    """The companion state of name.bin is returned only while name.bin is
+    the file it was written with (its SHA-256), and only in its own format."""
+    name = str(tmp_path / "f")
+    file_io.write_vec(name + ".bin", np.arange(6.0), 2)
+    file_io._write_state(name, {"time_elapsed": 0.125})
+    st = file_io._read_state(name)
+    assert st["time_elapsed"] == 0.125 and st["format"] == "bolt_amd restart state 1"
+    file_io.write_vec(name + ".bin", np.arange(6.0) + 1.0, 2)
+    assert file_io._read_state(name) is None
+    file_io._write_state(name, {"time_elapsed": 0.5})
+    assert file_io._read_state(name)["time_elapsed"] == 0.5
+    open(name + ".state.json", "w").write("{not json")
+    assert file_io._read_state(name) is None
+    assert file_io._read_state(str(tmp_path / "missing")) is None
