@@ -16,7 +16,8 @@ LAYOUT = {"reference": 0, "q1-lines": 1}
 BC = {"periodic": 0, "none": 0, "mirror": 1, "dirichlet": 2, "mirror+dirichlet": 3}
 STEPPER = {"RK2": 0, "SSP-RK3": 1}
 
-EXPORTS = ("boltx_fvm_stage", "boltx_fvm_stage_noalloc", "boltx_moments",
+EXPORTS = ("boltx_fvm_stage", "boltx_fvm_stage_noalloc", "boltx_fvm_stage_coeffs",
+           "boltx_moments",
            "boltx_workspace_size", "boltx_workspace_init",
            "boltx_currents", "boltx_fdtd_step", "boltx_yee_to_cell_centered",
            "boltx_check_finite", "boltx_adjoint_stage", "boltx_adjoint_workspace_size",
@@ -54,6 +55,16 @@ class Stage(ctypes.Structure):
                 ("mom_in", ctypes.c_void_p), ("mom_out", ctypes.c_void_p),
                 ("fields_c_p1", ctypes.c_void_p), ("fields_c_p2", ctypes.c_void_p),
                 ("fields_c_p3", ctypes.c_void_p), ("workspace", ctypes.c_void_p)]
+
+
+class Coeff(ctypes.Structure):
+    _fields_ = [("ptr", ctypes.c_void_p), ("stride_p", ctypes.c_int64),
+                ("stride_q1", ctypes.c_int64), ("stride_q2", ctypes.c_int64)]
+
+
+class Coeffs(ctypes.Structure):
+    _fields_ = [("c_q1", Coeff), ("c_q2", Coeff), ("c_p1", Coeff), ("c_p2", Coeff),
+                ("c_p3", Coeff), ("source", Coeff), ("tau", Coeff)]
 
 
 class Adjoint(ctypes.Structure):
@@ -94,6 +105,7 @@ def load(path=LIB_PATH):
     g, ph, st = ctypes.POINTER(Grid), ctypes.POINTER(Physics), ctypes.POINTER(Stage)
     lib.boltx_fvm_stage.argtypes = [g, ph, st, vp]
     lib.boltx_fvm_stage_noalloc.argtypes = [g, ph, st, vp]
+    lib.boltx_fvm_stage_coeffs.argtypes = [g, ph, st, ctypes.POINTER(Coeffs), vp]
     lib.boltx_moments.argtypes = [g, ph, vp, vp, vp, vp]
     lib.boltx_workspace_size.argtypes = [g, ph]
     lib.boltx_workspace_init.argtypes = [g, ph, vp, vp]

@@ -35,6 +35,9 @@ def _check_solver(s):
         raise NotImplementedError('the adjoint needs periodic q boundaries')
     if s.N_species != 1:
         raise NotImplementedError('the adjoint is provided for one species')
+    if s._general:
+        raise NotImplementedError('the adjoint differentiates the built-in C_q and BGK with a '
+                                  'uniform tau')
 
 
 def interior_to_internal(s, x):

@@ -116,7 +116,7 @@ __global__ __launch_bounds__(kBlock) void fvm_stage_kernel(StageParams sp) {
   c.e1 = c.e2 = c.e3 = tab;
   if (sp.source) build_tables(sp, q, tab, c);
   if constexpr (PADV) {
-    qcell_fields(sp, c);
+    if (!sp.cp1.ptr) qcell_fields(sp, c);
     const T* u = (const T*)sp.u + (long)q * sp.P;
     for (int p = threadIdx.x; p < sp.P; p += blockDim.x) blk[p] = u[p];
   }
@@ -755,6 +755,75 @@ extern "C" int boltx_fvm_stage(const boltx_grid* grid, const boltx_physics* phys
   int rc = check_stage(grid, phys, st);
   if (rc) return rc;
   return fvm_stage_impl(grid, phys, st, st->workspace, (hipStream_t)stream);
+}
+
+static Coeff coeff_of(const boltx_coeff& c) {
+  Coeff k;
+  k.ptr = c.ptr;
+  k.sp = (long)c.stride_p;
+  k.s1 = (long)c.stride_q1;
+  k.s2 = (long)c.stride_q2;
+  return k;
+}
+
+extern "C" int boltx_fvm_stage_coeffs(const boltx_grid* grid, const boltx_physics* phys,
+                                      const boltx_stage* st, const boltx_coeffs* co,
+                                      void* stream) {
+  if (!co) return fail(BOLTX_EINVAL, "coeffs is NULL");
+  if (!grid) return fail(BOLTX_EINVAL, "grid is NULL");
+  if (grid->layout != BOLTX_LAYOUT_REFERENCE)
+    return fail(BOLTX_EUNSUPPORTED, "caller-supplied terms need the reference layout");
+  const bool user_cp = co->c_p1.ptr || co->c_p2.ptr || co->c_p3.ptr;
+  if (user_cp && !(co->c_p1.ptr && co->c_p2.ptr && co->c_p3.ptr))
+    return fail(BOLTX_EINVAL, "c_p1, c_p2 and c_p3 are given together");
+  if (user_cp && (!phys || !phys->p_advection))
+    return fail(BOLTX_EINVAL, "c_p needs p_advection");
+  if (co->source.ptr && co->tau.ptr)
+    return fail(BOLTX_EINVAL, "tau applies to the built-in source, not to a caller's source");
+  // with the caller's source no moment block of u is read, and with the
+  // caller's C_p no fields: check the stage as if both were there
+  boltx_stage chk = *st;
+  if (co->source.ptr && !chk.mom_in) chk.mom_in = (const double*)co->source.ptr;
+  if (user_cp)
+    chk.fields_c_p1 = chk.fields_c_p2 = chk.fields_c_p3 = co->c_p1.ptr;
+  int rc = check_stage(grid, phys, &chk);
+  if (rc) return rc;
+  StageParams sp = base_params(grid);
+  sp.riemann_q = phys->riemann_q;
+  sp.riemann_p = phys->riemann_p;
+  // sp.source turns the built-in BGK on; a caller's source replaces it
+  sp.source = phys->source_enabled && !co->source.ptr;
+  sp.p_dim = phys->p_dim;
+  sp.mass = phys->mass;
+  sp.charge = phys->charge;
+  sp.kB = phys->boltzmann_constant;
+  sp.inv_tau = phys->source_enabled && !co->tau.ptr ? 1.0 / phys->tau : 0.0;
+  sp.a = st->a;
+  sp.b = st->b;
+  sp.cdt = st->cdt;
+  sp.u = st->u;
+  sp.w = st->w;
+  sp.un = st->un;
+  sp.out = st->out;
+  sp.mom_in = st->mom_in;
+  sp.mom_out = st->mom_out;
+  sp.fp1 = st->fields_c_p1;
+  sp.fp2 = st->fields_c_p2;
+  sp.fp3 = st->fields_c_p3;
+  sp.cq1 = coeff_of(co->c_q1);
+  sp.cq2 = coeff_of(co->c_q2);
+  sp.cp1 = coeff_of(co->c_p1);
+  sp.cp2 = coeff_of(co->c_p2);
+  sp.cp3 = coeff_of(co->c_p3);
+  sp.src = coeff_of(co->source);
+  sp.tau = coeff_of(co->tau);
+  // the built-in BGK needs the Maxwellian of u; a caller's source does not
+  if (co->source.ptr) sp.mom_in = nullptr;
+  hipStream_t s = (hipStream_t)stream;
+  const bool padv = phys->p_advection != 0;
+  if (phys->precision == BOLTX_FP64)
+    return launch_stage_rq<double>(sp, phys->recon_q, phys->recon_p, padv, s);
+  return launch_stage_rq<float>(sp, phys->recon_q, phys->recon_p, padv, s);
 }
 
 extern "C" int boltx_moments(const boltx_grid* grid, const boltx_physics* phys, const void* f,

@@ -210,6 +210,28 @@ __host__ __device__ inline T flux_diff(const T a[7], T C, int riemann, T* left_s
   return C * sR - C * sL;
 }
 
+// Flux difference F(i+1/2) - F(i-1/2) as flux_diff, with face velocities
+// that differ between the two faces (a caller-supplied C: CL at the left
+// face i-1/2, CR at the right face i+1/2), df_dt_fvm.py:67-89 and
+// riemann.py:46-49 with C evaluated per face.  Both states of each face are
+// reconstructed.
+template <int R, typename T>
+__host__ __device__ inline T flux_diff_var(const T a[7], T CL, T CR, int riemann) {
+  const T mL = face_value<R>(a[0], a[1], a[2], a[3], a[4], true);
+  const T pL = face_value<R>(a[1], a[2], a[3], a[4], a[5], false);
+  const T mR = face_value<R>(a[1], a[2], a[3], a[4], a[5], true);
+  const T pR = face_value<R>(a[2], a[3], a[4], a[5], a[6], false);
+  T FL, FR;
+  if (riemann == RS_LLF) {
+    FL = T(0.5) * (CL * (mL + pL) - aabs(CL) * (pL - mL));
+    FR = T(0.5) * (CR * (mR + pR) - aabs(CR) * (pR - mR));
+  } else {
+    FL = CL * (CL > T(0) ? mL : pL);
+    FR = CR * (CR > T(0) ? mR : pR);
+  }
+  return FR - FL;
+}
+
 // collision_operator.py:102-116: bulk velocity and temperature from the raw
 // moments (n, n*u1, n*u2, n*u3, E) of one q-cell, with the 1e-30 floors.
 struct BgkCell {

@@ -12,6 +12,20 @@
 
 namespace bx {
 
+// A term supplied by the caller instead of the built-in one (a user
+// callback evaluated on the device): the value at p-cell p of interior
+// q-cell (i1, i2) is ptr[p*sp + i1*s1 + i2*s2] (element strides; a stride of 0
+// broadcasts along that index).  Face arrays are read at i1 = n_q1 (i2 =
+// n_q2) as well: the left face of the cell after the last interior one.
+struct Coeff {
+  const double* ptr;  // NULL: the built-in term
+  long sp, s1, s2;
+};
+
+__host__ __device__ inline double coeff_at(const Coeff& c, long p, long i1, long i2) {
+  return c.ptr[p * c.sp + i1 * c.s1 + i2 * c.s2];
+}
+
 struct StageParams {
   int n_q1, n_q2, n_p1, n_p2, n_p3, P, Q;
   double p1s, p2s, p3s, dp1, dp2, dp3;
@@ -32,6 +46,10 @@ struct StageParams {
   // non-periodic q axes: ghost q-rows follow the Q interior rows of every
   // f-sized array (left, right, bottom, top blocks of ng rows per q-line)
   int ng, gh1, gh2;
+  // caller-supplied terms (generic kernel only): C_q1 at left q1-faces, C_q2
+  // at bottom q2-faces, C_p1..3 at the left / bottom / back p-faces, an
+  // additive source replacing BGK, the BGK collision time per cell
+  Coeff cq1, cq2, cp1, cp2, cp3, src, tau;
 };
 
 // Per-q-cell state the stage needs besides f: the stencil neighbours of the
@@ -112,6 +130,24 @@ __host__ __device__ inline T p_flux_diff(const T* blk, int p, int j, int n, int 
   return flux_diff<RP, T>(a, C, riemann, (T*)nullptr);
 }
 
+// The same with a caller-supplied C_p along the axis: C at the cell's own
+// lower face and at the lower face of the next cell along the axis (the
+// periodic shift of df_dt_fvm.py:306-308 wraps the last cell's upper face to
+// the first cell's lower face).
+template <int RP, typename T>
+__host__ __device__ inline T p_flux_diff_coeff(const T* blk, int p, int j, int n, int stride,
+                                               const Coeff& C, int i1, int i2, int riemann) {
+  T a[7];
+  for (int k = 0; k < 7; ++k) {
+    const int jj = ((j + k - 3) % n + n) % n;
+    a[k] = blk[p + (jj - j) * stride];
+  }
+  const int jr = j + 1 == n ? 0 : j + 1;
+  const T CL = T(coeff_at(C, p, i1, i2));
+  const T CR = T(coeff_at(C, p + (jr - j) * stride, i1, i2));
+  return flux_diff_var<RP, T>(a, CL, CR, riemann);
+}
+
 // The stage value of phase-space cell (q, p).  ``blk`` is the q-cell's P
 // values of u (needed only when p_advection is on).
 template <typename T, int RQ, int RP, bool PADV>
@@ -127,29 +163,59 @@ __host__ __device__ inline T stage_cell(const StageParams& sp, const QCell<T>& c
   v[2] = centre(sp.p3s, j3, sp.dp3);
   const long P = sp.P;
   const T f = u[(long)c.q * P + p];
+  const int i1 = c.q % sp.n_q1, i2 = c.q / sp.n_q1;
   constexpr int r = recon_reach(RQ) + 1;  // neighbours read on each side
   T d = T(0);
-  // q1 and q2 fluxes; C_q1 = v1, C_q2 = v2 (advection_terms.py:64-97)
+  // q1 and q2 fluxes; built-in C_q1 = v1, C_q2 = v2 (advection_terms.py:64-97),
+  // else the caller's face values
   T t1 = T(0), t2 = T(0);
   if (sp.n_q1 > 1) {
     T a[7];
+    for (int k = 0; k < 7; ++k) a[k] = T(0);
     for (int k = 3 - r; k <= 3 + r; ++k) a[k] = k == 3 ? f : u[(long)c.nb1[k] * P + p];
-    t1 = flux_diff<RQ, T>(a, T(v[0]), sp.riemann_q, (T*)nullptr);
+    if (sp.cq1.ptr)
+      t1 = flux_diff_var<RQ, T>(a, T(coeff_at(sp.cq1, p, i1, i2)),
+                                T(coeff_at(sp.cq1, p, i1 + 1, i2)), sp.riemann_q);
+    else
+      t1 = flux_diff<RQ, T>(a, T(v[0]), sp.riemann_q, (T*)nullptr);
   }
   if (sp.n_q2 > 1) {
     T a[7];
+    for (int k = 0; k < 7; ++k) a[k] = T(0);
     for (int k = 3 - r; k <= 3 + r; ++k) a[k] = k == 3 ? f : u[(long)c.nb2[k] * P + p];
-    t2 = flux_diff<RQ, T>(a, T(v[1]), sp.riemann_q, (T*)nullptr);
+    if (sp.cq2.ptr)
+      t2 = flux_diff_var<RQ, T>(a, T(coeff_at(sp.cq2, p, i1, i2)),
+                                T(coeff_at(sp.cq2, p, i1, i2 + 1)), sp.riemann_q);
+    else
+      t2 = flux_diff<RQ, T>(a, T(v[1]), sp.riemann_q, (T*)nullptr);
   }
   d = -t1 * T(sp.inv_dq1) - t2 * T(sp.inv_dq2);
-  if (sp.source) {
+  if (sp.src.ptr) {
+    d = d + T(coeff_at(sp.src, p, i1, i2));  // df_dt_fvm.py:91-97, the caller's source
+  } else if (sp.source) {
     // f_MB = ((n c^(p_dim/2) * e1) * e2) * e3 (collision_operator.py:52-65)
     T fmb = T(c.pref) * c.e1[j1];
     if (sp.p_dim >= 2) fmb = fmb * c.e2[j2];
     if (sp.p_dim >= 3) fmb = fmb * c.e3[j3];
-    d = d + (fmb - f) * T(sp.inv_tau);
+    if (sp.tau.ptr)
+      d = d + (fmb - f) / T(coeff_at(sp.tau, p, i1, i2));  // collision_operator.py:119-128
+    else
+      d = d + (fmb - f) * T(sp.inv_tau);
   }
-  if constexpr (PADV) {
+  if constexpr (PADV) if (sp.cp1.ptr) {
+    // the caller's C_p1..3 at the p-faces (df_dt_fvm.py:251-321)
+    T dp = T(0);
+    if (sp.n_p1 > 1)
+      dp = p_flux_diff_coeff<RP, T>(blk, p, j1, sp.n_p1, 1, sp.cp1, i1, i2, sp.riemann_p) *
+           T(sp.inv_dp1);
+    if (sp.n_p2 > 1)
+      dp = dp + p_flux_diff_coeff<RP, T>(blk, p, j2, sp.n_p2, sp.n_p1, sp.cp2, i1, i2,
+                                         sp.riemann_p) * T(sp.inv_dp2);
+    if (sp.n_p3 > 1)
+      dp = dp + p_flux_diff_coeff<RP, T>(blk, p, j3, sp.n_p3, sp.n_p1 * sp.n_p2, sp.cp3, i1,
+                                         i2, sp.riemann_p) * T(sp.inv_dp3);
+    d = d - dp;
+  } else {
     // C_p at the p-faces, cell-centred fields (advection_terms.py:184-186)
     const double em = sp.charge / sp.mass;
     T dp = T(0);

@@ -42,6 +42,9 @@ class Ensemble:
         for s in self.solvers:
             if s.N_q2 != 1 or s._fields_on:
                 raise NotImplementedError('ensemble members need N_q2 = 1 and fields off')
+            if s._general:
+                raise NotImplementedError('ensemble members use the built-in C_q and BGK '
+                                          'with a uniform tau')
             if s._stepper != s0._stepper or s._precision != s0._precision \
                     or s._recon_q != s0._recon_q:
                 raise NotImplementedError('ensemble members must share stepper, precision '

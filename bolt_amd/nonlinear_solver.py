@@ -38,6 +38,28 @@ from .src.nonrelativistic_boltzmann import moments as _mom
 
 F64 = torch.float64
 
+_PLUGIN_PKG = 'nonrelativistic_boltzmann'
+
+
+def _is_builtin(fn, builtin, module):
+    """True for the built-in plugin function ``builtin`` itself, or for a
+    function of the same name from a module of the same dotted name ending
+    (``...nonrelativistic_boltzmann.<module>``): the reference's own plugin
+    module (bolt/src/nonrelativistic_boltzmann) or a copy of it, which the
+    fused kernels evaluate in-register."""
+    if fn is builtin:
+        return True
+    mod = getattr(fn, '__module__', None) or ''
+    suffix = _PLUGIN_PKG + '.' + module
+    return getattr(fn, '__name__', None) == builtin.__name__ and \
+        (mod == suffix or mod.endswith('.' + suffix))
+
+
+def _is_builtin_module(m, builtin):
+    name = getattr(m, '__name__', '') or ''
+    suffix = _PLUGIN_PKG + '.' + builtin.__name__.rsplit('.', 1)[-1]
+    return m is builtin or name == suffix or name.endswith('.' + suffix)
+
 
 class _Species(object):
     """The state one species needs on the device: its grid (own p-range and
@@ -50,6 +72,7 @@ class _Species(object):
         self.grid = self.phys = self.f = self.mom = self.ws = None
         self.bufs = {}
         self.mass = self.charge = self.tau = 1.0
+        self.tau_field = None  # (P, S, Q1g, Q2g) FP64 when tau varies over the grid
         self.dp = [1.0, 1.0, 1.0]
 
 
@@ -138,6 +161,11 @@ class nonlinear_solver(object):
 
         self._configure(ps, p)
         self._build_grids(ps)
+        if self._general and ((self.N_q1 > 1 and self.N_g1 < 1) or
+                              (self.N_q2 > 1 and self.N_g2 < 1)):
+            raise NotImplementedError('user-supplied terms need ghost zones on every q axis '
+                                      'with more than one cell (their face values reach one '
+                                      'cell past the interior)')
         for k, sp in enumerate(self._spc):
             sp.grid = _lib.Grid(self.N_q1, self.N_q2, self.N_p1, self.N_p2, self.N_p3, 1,
                                 self._layout, float(self.dq1), float(self.dq2), sp.dp[0],
@@ -205,11 +233,15 @@ class nonlinear_solver(object):
         lay = getattr(p, 'memory_layout', 'auto')
         fields = bool(getattr(p, 'fields_enabled', False))
         ghosted = any(self._ghosted)
+        self._plugin_flags(ps, p)
         if lay == 'auto':
-            lay = 'q1-lines' if (not fields and not ghosted and ps.N_q1 in (64, 128, 256)) \
-                else 'reference'
+            lay = 'q1-lines' if (not fields and not ghosted and ps.N_q1 in (64, 128, 256)
+                                 and not self._general) else 'reference'
         if lay not in _lib.LAYOUT:
             raise NotImplementedError("memory_layout must be 'auto', 'reference' or 'q1-lines'")
+        if lay == 'q1-lines' and self._general:
+            raise NotImplementedError('user-supplied C_q, C_p, source or a varying tau need '
+                                      'the reference layout')
         if lay == 'q1-lines' and (fields or ps.N_q1 > 256):
             raise NotImplementedError('the q1-line layout needs fields off and N_q1 <= 256')
         if lay == 'q1-lines' and ghosted:
@@ -227,15 +259,6 @@ class nonlinear_solver(object):
         self._layout = _lib.LAYOUT[lay]
         if self._stepper not in ('RK2', 'SSP-RK3'):
             raise NotImplementedError('time_stepper_fvm must be RK2 or SSP-RK3')
-        if ps.C_q is not _adv.C_q:
-            raise NotImplementedError('the fused kernel implements the built-in C_q '
-                                      '(bolt_amd.src.nonrelativistic_boltzmann)')
-        self._source_on = bool(getattr(p, 'source_enabled', False))
-        if self._source_on:
-            if ps.source is not _coll.BGK:
-                raise NotImplementedError('the fused kernel implements the built-in BGK source')
-            if ps.moments is not _mom:
-                raise NotImplementedError('BGK needs the built-in moments module')
         self._fields_on = bool(getattr(p, 'fields_enabled', False))
         self._p_adv = self._fields_on and getattr(p, 'solver_method_in_p', 'FVM') == 'FVM'
         self._es = False
@@ -253,8 +276,29 @@ class nonlinear_solver(object):
                 raise NotImplementedError('fields_type %r is not provided' % ftype)
             if getattr(p, 'hybrid_model_enabled', False):
                 raise NotImplementedError('the hybrid model is not provided')
-            if self._p_adv and ps.C_p is not _adv.C_p:
-                raise NotImplementedError('the fused kernel implements the built-in C_p')
+            if self._user_cq and ftype == 'electrodynamic':
+                raise NotImplementedError('the currents of the FDTD update take their face '
+                                          'states from the built-in C_q')
+
+    def _plugin_flags(self, ps, p):
+        """Which terms are the built-in plugins (bolt_amd.src or the
+        reference's bolt.src.nonrelativistic_boltzmann modules: evaluated
+        in-register by the fused kernels) and which are the user's callbacks
+        (evaluated with torch on the device each RHS and fed to the generic
+        stage kernel, df_dt_fvm.py:67-97, 251-267).  A collision time that
+        varies over the grid also takes the generic kernel."""
+        self._source_on = bool(getattr(p, 'source_enabled', False))
+        fields = bool(getattr(p, 'fields_enabled', False))
+        self._user_cq = not _is_builtin(ps.C_q, _adv.C_q, 'advection_terms')
+        self._user_cp = fields and getattr(p, 'solver_method_in_p', 'FVM') == 'FVM' and \
+            not _is_builtin(ps.C_p, _adv.C_p, 'advection_terms')
+        self._user_src = self._source_on and \
+            not _is_builtin(ps.source, _coll.BGK, 'collision_operator')
+        self._builtin_moments = _is_builtin_module(ps.moments, _mom)
+        if self._source_on and not self._user_src and not self._builtin_moments:
+            raise NotImplementedError('the built-in BGK source needs the built-in moments module')
+        self._tau_varies = False  # set by _build_grids
+        self._general = self._user_cq or self._user_cp or self._user_src
 
     def _build_grids(self, ps):
         """calculate_q.py:7-41 and calculate_p.py:8-57 as device tensors."""
@@ -299,10 +343,13 @@ class nonlinear_solver(object):
                                 device=dev).reshape(1, S, 1, 1)
         self.dp3 = torch.tensor([sp.dp[2] for sp in self._spc], dtype=F64,
                                 device=dev).reshape(1, S, 1, 1)
-        # collision time: params.tau(q1, q2, p1, p2, p3) must be uniform per species
+        # collision time of the built-in BGK: params.tau(q1, q2, p1, p2, p3)
+        # (collision_operator.py:119), which has no time argument, evaluated
+        # once; uniform per species -> a scalar for the fused kernels, else a
+        # per-cell field for the generic kernel
         for sp in self._spc:
-            sp.tau = 1.0
-        if self._source_on:
+            sp.tau, sp.tau_field = 1.0, None
+        if self._source_on and not self._user_src:
             tau = self.params.tau(self.q1_center, self.q2_center, self.p1_center,
                                   self.p2_center, self.p3_center)
             tau = torch.as_tensor(tau, dtype=F64, device=dev)
@@ -310,9 +357,17 @@ class nonlinear_solver(object):
                                      tuple(self.q1_center.shape[2:]))
             for k, sp in enumerate(self._spc):
                 t0 = float(tau[:, k].reshape(-1)[0])
-                if not bool((tau[:, k] == t0).all()):
-                    raise NotImplementedError('the fused kernel needs a uniform tau per species')
                 sp.tau = t0
+                if not bool((tau[:, k] == t0).all()):
+                    sp.tau_field = tau
+                    self._tau_varies = True
+            if self._tau_varies:
+                self._general = True
+                if self._layout == _lib.LAYOUT['q1-lines']:
+                    if getattr(self.params, 'memory_layout', 'auto') == 'q1-lines':
+                        raise NotImplementedError('a collision time that varies over the grid '
+                                                  'needs the reference layout')
+                    self._layout = _lib.LAYOUT['reference']
 
     # -- layout helpers -------------------------------------------------------
     def _q_interior(self, x):
@@ -562,13 +617,82 @@ class nonlinear_solver(object):
                 fs.stage_fields = fs._cc_at_n_plus_half
         elif fs.at_n is False:
             fs.evolve_electrodynamic_fields(self._currents(us, 'J_sum'), self.dt)
+        if self._user_cp:
+            # df_dt_fvm.py:251-267: the user's C_p at the p-faces, each call
+            # taking the fields through fields_solver.get_fields()
+            t, p = self.time_elapsed, self.params
+            self._cp_user = [
+                self._C_p(t, self.q1_center, self.q2_center, self.p1_left, self.p2_left,
+                          self.p3_left, fs, p)[0],
+                self._C_p(t, self.q1_center, self.q2_center, self.p1_bottom, self.p2_bottom,
+                          self.p3_bottom, fs, p)[1],
+                self._C_p(t, self.q1_center, self.q2_center, self.p1_back, self.p2_back,
+                          self.p3_back, fs, p)[2]]
+            return F
         return [fs._take_for_C_p() for _ in range(3)]  # df_dt_fvm.py:251-267
 
-    def _stage_kernel(self, u, w, un, out, a, b, cdt, mom_in, mom_out, F):
+    # -- user callbacks (generic stage kernel) ---------------------------------
+    def _coeff(self, x, k, keep):
+        """boltx_coeff of species k of a callback result ``x`` (broadcastable
+        to the reference's (P, S, Q1g, Q2g)), with its origin at the first
+        interior q-cell; broadcast axes keep a zero stride.  The FP64 view is
+        appended to ``keep`` so it outlives the launch."""
+        x = torch.as_tensor(x, dtype=F64, device=self.device)
+        full = (self._P, self.N_species) + tuple(self.q1_center.shape[2:])
+        v = torch.broadcast_to(x, full)[:, k]
+        keep.append(v)
+        off = v.stride(1) * self.N_g1 + v.stride(2) * self.N_g2
+        return _lib.Coeff(v.data_ptr() + v.element_size() * off, v.stride(0), v.stride(1),
+                          v.stride(2))
+
+    def _callback_terms(self, us):
+        """The user's terms of one RHS evaluation with stage inputs ``us``
+        (one per species), as one boltx_coeffs per species: C_q at the left
+        and bottom q-faces (df_dt_fvm.py:67-77), the source on the stage input
+        (:91-97), the C_p of this RHS (from _fields_for_stage) and a varying
+        collision time.  Also returns the arrays to keep alive."""
+        t, p = self.time_elapsed, self.params
+        keep = []
+        per = [_lib.Coeffs() for _ in self._spc]
+        if self._user_cq:
+            C1 = self._C_q(t, self.q1_left_center, self.q2_left_center, self.p1_center,
+                           self.p2_center, self.p3_center, p)[0]
+            C2 = self._C_q(t, self.q1_center_bot, self.q2_center_bot, self.p1_center,
+                           self.p2_center, self.p3_center, p)[1]
+            for k, c in enumerate(per):
+                c.c_q1, c.c_q2 = self._coeff(C1, k, keep), self._coeff(C2, k, keep)
+        if self._user_src:
+            parts = []
+            for k in self._species_loop():
+                self._fill_ghosts(us[k])
+                parts.append(self._to_reference(us[k]))
+            f_ref = torch.cat(parts, dim=1).to(F64)
+            S = self._source(f_ref, t, self.q1_center, self.q2_center, self.p1_center,
+                             self.p2_center, self.p3_center, self.compute_moments, p, False)
+            for k, c in enumerate(per):
+                c.source = self._coeff(S, k, keep)
+        if self._user_cp:
+            for k, c in enumerate(per):
+                c.c_p1, c.c_p2, c.c_p3 = (self._coeff(x, k, keep) for x in self._cp_user)
+        for k, (c, sp) in enumerate(zip(per, self._spc)):
+            if sp.tau_field is not None:
+                c.tau = self._coeff(sp.tau_field, k, keep)
+        return per, keep
+
+    def _stage_kernel(self, u, w, un, out, a, b, cdt, mom_in, mom_out, F, co=None):
         """The current species' stage launch, out = a*un + b*(w + cdt*L(u)),
         with its ghost refresh (fvm_operator.py:22-33) and the moments of
-        out."""
+        out; ``co`` (boltx_coeffs) carries the user's terms to the generic
+        kernel."""
         self._fill_ghosts(u)
+        if co is not None:
+            st = _lib.Stage(_lib.ptr(u), _lib.ptr(w), _lib.ptr(un), _lib.ptr(out),
+                            float(a), float(b), float(cdt), _lib.ptr(mom_in), _lib.ptr(mom_out),
+                            _lib.ptr(F[0]), _lib.ptr(F[1]), _lib.ptr(F[2]), None)
+            _lib.check(self._L.boltx_fvm_stage_coeffs(
+                ctypes.byref(self._grid), ctypes.byref(self._phys), ctypes.byref(st),
+                ctypes.byref(co), _lib.stream_handle()))
+            return
         # In the q1-line layout the moments of the output are a separate
         # streaming pass; issue it as its own call.
         split = self._layout == _lib.LAYOUT['q1-lines'] and mom_out is not None
@@ -596,7 +720,8 @@ class nonlinear_solver(object):
         (df_dt_fvm.py:33-339 inside fvm_operator.py:22-33)."""
         self._stage_moms = [mom_in]
         F = self._fields_for_stage([u])
-        self._stage_kernel(u, w, un, out, a, b, cdt, mom_in, mom_out, F)
+        co = self._callback_terms([u])[0][0] if self._general else None
+        self._stage_kernel(u, w, un, out, a, b, cdt, mom_in, mom_out, F, co)
 
     def _advance(self, dt):
         """Advance f by dt without the host-side divergence check (the device
@@ -629,9 +754,12 @@ class nonlinear_solver(object):
                 args.append([arr(x) for x in (u, w, un, out, mi, mo)])
             self._stage_moms = [x[4] for x in args]
             F = self._fields_for_stage([x[0] for x in args], stage)
+            co = [None] * len(self._spc)
+            if self._general:
+                co, keep = self._callback_terms([x[0] for x in args])
             for k in self._species_loop():
                 uu, ww, nn, oo, mmi, mmo = args[k]
-                self._stage_kernel(uu, ww, nn, oo, a, b, cdt, mmi, mmo, F)
+                self._stage_kernel(uu, ww, nn, oo, a, b, cdt, mmi, mmo, F, co[k])
         for _ in self._species_loop():
             _lib.check(self._L.boltx_check_finite(ctypes.byref(self._grid), _lib.ptr(self._mom),
                                                   _lib.ptr(self._flag), _lib.stream_handle()))
@@ -667,8 +795,7 @@ class nonlinear_solver(object):
         """Moments shaped (1, N_s, Q1g, Q2g).  The built-in five come from the
         HIP reduction; other moment functions of the user's module are
         evaluated with torch on the device."""
-        if f is None and moment_name in _mom._BUILTIN and \
-                self.physical_system.moments is _mom:
+        if f is None and moment_name in _mom._BUILTIN and self._builtin_moments:
             k = _mom._BUILTIN.index(moment_name)
             return torch.cat([self._q_padded(sp.mom[k]) for sp in self._spc], dim=1)
         fn = getattr(self.physical_system.moments, moment_name)

@@ -138,6 +138,47 @@ int boltx_fvm_stage(const boltx_grid *grid, const boltx_physics *phys,
 int boltx_fvm_stage_noalloc(const boltx_grid *grid, const boltx_physics *phys,
                             const boltx_stage *stage, void *stream);
 
+/* Caller-supplied terms of the stage: user callbacks (C_q, C_p, source,
+ * tau of the reference's physical_system plugins) evaluated on the device.
+ * A coefficient is an FP64 array read at p-cell p of interior q-cell
+ * (i1, i2) as ptr[p*stride_p + i1*stride_q1 + i2*stride_q2] (element
+ * strides; 0 broadcasts, so a callback's broadcast result is passed
+ * without materialising it); ptr = NULL selects the built-in term. */
+typedef struct boltx_coeff {
+  const double *ptr;
+  int64_t stride_p, stride_q1, stride_q2;
+} boltx_coeff;
+
+typedef struct boltx_coeffs {
+  /* C_q1 at the left q1-face of cell (i1, i2), read for i1 = 0..n_q1 (the
+   * C_q callback at q1_left_center, q2_left_center: df_dt_fvm.py:67-71) */
+  boltx_coeff c_q1;
+  /* C_q2 at the bottom q2-face, read for i2 = 0..n_q2 (at q1_center_bot,
+   * q2_center_bot: df_dt_fvm.py:73-77) */
+  boltx_coeff c_q2;
+  /* C_p1, C_p2, C_p3 at the left, bottom and back p-faces of every p-cell
+   * (the C_p callback at p1_left / p2_bottom / p3_back: df_dt_fvm.py:251-267);
+   * all three or none, with p_advection on */
+  boltx_coeff c_p1, c_p2, c_p3;
+  /* an additive source term evaluated on the stage input, applied in place
+   * of the built-in BGK (the source callback: df_dt_fvm.py:91-97) */
+  boltx_coeff source;
+  /* the built-in BGK's collision time per cell (params.tau(q1, q2, p1, p2,
+   * p3): collision_operator.py:119-128), instead of phys->tau */
+  boltx_coeff tau;
+} boltx_coeffs;
+
+/* boltx_fvm_stage with caller-supplied terms (reference layout; the generic
+ * one-workgroup-per-q-cell kernel).  Replaces df_dt_fvm
+ * (df_dt_fvm.py:33-339) for any advection_terms / source module, inside one
+ * stage of timestep_fvm (fvm_operator.py:22-33).  With the caller's source,
+ * mom_in is not read; with the caller's C_p, the fields are not read.  The
+ * caller keeps every coefficient array alive and unchanged until the stage
+ * has run on the stream; the indices above must stay inside them. */
+int boltx_fvm_stage_coeffs(const boltx_grid *grid, const boltx_physics *phys,
+                           const boltx_stage *stage, const boltx_coeffs *coeffs,
+                           void *stream);
+
 /* The moment block of f.  Replaces compute_moments
  * (bolt/lib/nonlinear/compute_moments.py:7-59) for density, mom_v1_bulk,
  * mom_v2_bulk, mom_v3_bulk and energy (moments.py:5-22), plus the BGK

@@ -140,9 +140,13 @@ def case_2d3v(seed=0, N_q=64, N_p=24, **over):
 
 
 def oracle_solver(case):
+    """The oracle for a case; a case's own ``advection`` module and
+    ``source`` function (user plugins written with torch ops) replace the
+    built-in ones."""
     from oracle import bolt_oracle as bo
     sysm = bo.make_system(case.domain, case.boundary_conditions, case.params, case.initialize,
-                          bo.advection_terms, bo.BGK, bo.moments)
+                          getattr(case, "advection", None) or bo.advection_terms,
+                          getattr(case, "source", None) or bo.BGK, bo.moments)
     return bo.OracleSolver(sysm)
 
 
@@ -153,7 +157,8 @@ def hip_solver(case):
     import bolt_amd.src.nonrelativistic_boltzmann.collision_operator as coll
     import bolt_amd.src.nonrelativistic_boltzmann.moments as mom
     system = physical_system(case.domain, case.boundary_conditions, case.params,
-                             case.initialize, adv, coll.BGK, mom)
+                             case.initialize, getattr(case, "advection", None) or adv,
+                             getattr(case, "source", None) or coll.BGK, mom)
     return nonlinear_solver(system)
 
 

@@ -22,6 +22,9 @@ def lib(contract="fast-honor-pragmas"):
         h.host_fvm_stage.argtypes = [ctypes.POINTER(L.Grid), ctypes.POINTER(L.Physics),
                                      ctypes.POINTER(L.Stage)]
         h.host_fvm_stage.restype = ctypes.c_int
+        h.host_fvm_stage_coeffs.argtypes = [ctypes.POINTER(L.Grid), ctypes.POINTER(L.Physics),
+                                            ctypes.POINTER(L.Stage), ctypes.POINTER(L.Coeffs)]
+        h.host_fvm_stage_coeffs.restype = ctypes.c_int
         _libs[contract] = h
     return _libs[contract]
 
@@ -76,4 +79,35 @@ def stage(o, u, w, un, a, b, cdt, precision=0, contract="fast-honor-pragmas", fi
     st = L.Stage(P(u), P(w), P(un), P(out), a, b, cdt, P(mom_in), P(mom_out),
                  P(F[0]), P(F[1]), P(F[2]), None)
     lib(contract).host_fvm_stage(ctypes.byref(g), ctypes.byref(ph), ctypes.byref(st))
+    return out, mom_out
+
+
+def coeff(o, x, keep):
+    """boltx_coeff of a callback result ``x`` broadcastable to the oracle's
+    (P, 1, Q1g, Q2g), origin at the first interior q-cell (the layout the
+    solver hands to boltx_fvm_stage_coeffs)."""
+    P = o.N_p1 * o.N_p2 * o.N_p3
+    x = torch.as_tensor(x, dtype=torch.float64)
+    v = torch.broadcast_to(x, (P, 1) + tuple(o.q1_center.shape[2:]))[:, 0]
+    keep.append(v)
+    off = v.stride(1) * o.N_g1 + v.stride(2) * o.N_g2
+    return L.Coeff(v.data_ptr() + 8 * off, v.stride(0), v.stride(1), v.stride(2))
+
+
+def stage_coeffs(o, u, w, un, a, b, cdt, co, precision=0, fields=None, **kw):
+    """One harness stage with caller-supplied terms ``co`` (boltx_coeffs)."""
+    g, ph = grid_physics(o, precision, **kw)
+    dt = np.float64 if precision == 0 else np.float32
+    u = np.ascontiguousarray(u, dtype=dt)
+    w = np.ascontiguousarray(w, dtype=dt)
+    un = None if un is None else np.ascontiguousarray(un, dtype=dt)
+    out = np.empty_like(u)
+    mom_in = moments_np(o, u)
+    mom_out = np.empty_like(mom_in)
+    P = lambda x: None if x is None else x.ctypes.data_as(ctypes.c_void_p)
+    F = [None, None, None] if fields is None else [np.ascontiguousarray(x) for x in fields]
+    st = L.Stage(P(u), P(w), P(un), P(out), a, b, cdt, P(mom_in), P(mom_out),
+                 P(F[0]), P(F[1]), P(F[2]), None)
+    lib().host_fvm_stage_coeffs(ctypes.byref(g), ctypes.byref(ph), ctypes.byref(st),
+                                ctypes.byref(co))
     return out, mom_out

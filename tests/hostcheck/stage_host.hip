@@ -54,7 +54,7 @@ static void run(StageParams sp) {
       c.e3 = tab.data() + sp.n_p1 + sp.n_p2;
     }
     const T* blk = (const T*)sp.u + (long)q * sp.P;
-    if (PADV) qcell_fields(sp, c);
+    if (PADV && !sp.cp1.ptr) qcell_fields(sp, c);
     double acc[5] = {0, 0, 0, 0, 0};
     for (int p = 0; p < sp.P; ++p) {
       double v[3];
@@ -95,6 +95,31 @@ extern "C" int host_fvm_stage(const boltx_grid* g, const boltx_physics* ph, cons
   sp.u = st->u; sp.w = st->w; sp.un = st->un; sp.out = st->out;
   sp.mom_in = st->mom_in; sp.mom_out = st->mom_out;
   sp.fp1 = st->fields_c_p1; sp.fp2 = st->fields_c_p2; sp.fp3 = st->fields_c_p3;
+  if (ph->precision == 0) run_rq<double>(sp, ph->recon_q, ph->recon_p, ph->p_advection != 0);
+  else run_rq<float>(sp, ph->recon_q, ph->recon_p, ph->p_advection != 0);
+  return 0;
+}
+
+static Coeff coeff_of(const boltx_coeff& c) {
+  Coeff k;
+  k.ptr = c.ptr; k.sp = (long)c.stride_p; k.s1 = (long)c.stride_q1; k.s2 = (long)c.stride_q2;
+  return k;
+}
+
+// The generic stage with caller-supplied terms (as boltx_fvm_stage_coeffs
+// sets up its parameters).
+extern "C" int host_fvm_stage_coeffs(const boltx_grid* g, const boltx_physics* ph,
+                                     const boltx_stage* st, const boltx_coeffs* co) {
+  StageParams sp = params_of(g, ph);
+  sp.a = st->a; sp.b = st->b; sp.cdt = st->cdt;
+  sp.u = st->u; sp.w = st->w; sp.un = st->un; sp.out = st->out;
+  sp.mom_in = st->mom_in; sp.mom_out = st->mom_out;
+  sp.fp1 = st->fields_c_p1; sp.fp2 = st->fields_c_p2; sp.fp3 = st->fields_c_p3;
+  sp.cq1 = coeff_of(co->c_q1); sp.cq2 = coeff_of(co->c_q2);
+  sp.cp1 = coeff_of(co->c_p1); sp.cp2 = coeff_of(co->c_p2); sp.cp3 = coeff_of(co->c_p3);
+  sp.src = coeff_of(co->source); sp.tau = coeff_of(co->tau);
+  sp.source = ph->source_enabled && !co->source.ptr;
+  sp.inv_tau = ph->source_enabled && !co->tau.ptr ? 1.0 / ph->tau : 0.0;
   if (ph->precision == 0) run_rq<double>(sp, ph->recon_q, ph->recon_p, ph->p_advection != 0);
   else run_rq<float>(sp, ph->recon_q, ph->recon_p, ph->p_advection != 0);
   return 0;
