@@ -699,8 +699,6 @@ static int check_stage(const boltx_grid* grid, const boltx_physics* phys, const 
     return fail(BOLTX_EINVAL, "q-grid smaller than the stencil reach");
   if (ghost_rows(grid) > 0 && grid->n_ghost < need)
     return fail(BOLTX_EINVAL, "n_ghost = %d is below the stencil reach %d", grid->n_ghost, need);
-  if (ghost_rows(grid) > 0 && phys->p_advection)
-    return fail(BOLTX_EUNSUPPORTED, "non-periodic q boundaries with p-advection");
   return BOLTX_OK;
 }
 

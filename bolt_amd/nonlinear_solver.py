@@ -247,8 +247,13 @@ class nonlinear_solver(object):
         if lay == 'q1-lines' and ghosted:
             raise NotImplementedError('non-periodic q boundaries need the reference layout')
         if ghosted:
-            if fields:
-                raise NotImplementedError('non-periodic q boundaries are provided with fields off')
+            if fields and getattr(p, 'fields_type', None) == 'electrodynamic':
+                # the reference's FDTD near a non-periodic boundary reads Yee ghost
+                # values that its circular shifts of the padded arrays evolve
+                # (fdtd_explicit.py:9-177) and that apply_bcs_fields only partly
+                # resets (fields/boundaries.py:419-567)
+                raise NotImplementedError('non-periodic q boundaries are provided with the '
+                                          'fields off or electrostatic')
             if getattr(p, 'p_space_grid', 'cartesian') != 'cartesian' and \
                     any(k in (1, 3) for k in self._bc):
                 raise NotImplementedError('Unsupported coordinate system in p_space')

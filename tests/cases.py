@@ -265,19 +265,23 @@ def case_bc(*, left="mirror", right="mirror", bottom="periodic", top="periodic",
             N_q1=8, N_q2=6, N_p1=6, N_p2=6, p_max=6.0, p_dim=2, N_ghost=3,
             recon_q="weno5", riemann="upwind-flux", stepper="RK2", source=False,
             tau=0.05, precision="fp64", drift=(0.7, -0.4), bump=0.3, inflow=None,
-            dt=2e-3):
+            dt=2e-3, electrostatic=False):
     """Non-periodic q boundaries (boundaries.py): a drifting Maxwellian with a
     Gaussian density bump centred in [0,1)^2 (width 0.2, amplitude ``bump``),
     mean velocity ``drift``, and per-side q-BCs; dirichlet sides inject a
     drifting Maxwellian (density, u1, u2) = ``inflow`` or (1.2, 0.5, 0.3)
     through f_left / f_right / f_bottom / f_top(f, t, q1, q2, p1, p2, p3,
-    params).  1-D problems (N_q2 = 1) take bottom = top = 'none'."""
+    params).  1-D problems (N_q2 = 1) take bottom = top = 'none'.  With
+    ``electrostatic`` the FFT Poisson fields of the electrons (charge -1)
+    drive p-advection."""
     if N_q2 == 1:
         bottom = top = "none"
+    es = dict(fields=True, fields_type="electrostatic", fields_initialize="fft") \
+        if electrostatic else {}
     c = maxwellian_case(N_q1=N_q1, N_q2=N_q2, N_p1=N_p1, N_p2=N_p2 if p_dim >= 2 else 1,
                         N_ghost=N_ghost, p_max=p_max, p_dim=p_dim, recon_q=recon_q,
                         recon_p=recon_q, riemann=riemann, stepper=stepper, tau=tau,
-                        source=source, precision=precision, memory_layout="auto")
+                        source=source, precision=precision, memory_layout="auto", **es)
     n_in, u1_in, u2_in = inflow or (1.2, 0.5, 0.3)
 
     def f_side(f, t, q1, q2, p1, p2, p3, params):
