@@ -12,10 +12,13 @@ moment-block / Maxwellian-table launch.
 The work-list shape (waves per workgroup, cells per wave task, persistent
 workgroups per CU, cells per moment workgroup) is picked by ``autotune``,
 which times candidate shapes on a scratch copy of the ensemble and caches the
-winner per ensemble signature.
+winner per ensemble signature, in the process and in a JSON file (per GPU
+model) that later processes read instead of tuning again.
 """
 import ctypes
 import itertools
+import json
+import os
 import time
 
 import torch
@@ -26,6 +29,53 @@ F64 = torch.float64
 
 DEFAULT_TUNING = (4, 1024, 4, 16384)
 _TUNED = {}
+# Tunings persist across processes in a JSON file (one entry per ensemble
+# signature and GPU model), by default under the user's cache directory.
+DEFAULT_TUNING_CACHE = os.path.join(os.environ.get('XDG_CACHE_HOME') or
+                                    os.path.join(os.path.expanduser('~'), '.cache'),
+                                    'bolt_amd', 'ensemble_tuning.json')
+
+
+def cache_key(signature, device_name):
+    """JSON key of a tuning: the GPU model and the ensemble signature."""
+    return device_name + '|' + json.dumps(signature)
+
+
+def load_tuning(path, key):
+    """The cached tuning for ``key`` in the JSON file ``path``, or None (no
+    file, an unreadable or malformed file, no entry, a malformed entry)."""
+    try:
+        with open(path) as fh:
+            table = json.load(fh)
+        t = table[key]
+        if isinstance(t, list) and len(t) == 4 and all(isinstance(x, int) for x in t):
+            return tuple(t)
+    except (OSError, ValueError, KeyError, TypeError):
+        pass
+    return None
+
+
+def store_tuning(path, key, tuning):
+    """Add ``key`` -> ``tuning`` to the JSON file ``path`` (written to a
+    temporary file and renamed, so a reader never sees a partial file).
+    Returns False if the file cannot be written."""
+    try:
+        try:
+            with open(path) as fh:
+                table = json.load(fh)
+            if not isinstance(table, dict):
+                table = {}
+        except (OSError, ValueError):
+            table = {}
+        table[key] = [int(x) for x in tuning]
+        os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+        tmp = '%s.%d.tmp' % (path, os.getpid())
+        with open(tmp, 'w') as fh:
+            json.dump(table, fh, indent=1)
+        os.replace(tmp, path)
+        return True
+    except OSError:
+        return False
 
 
 def _tuning(t):
@@ -33,7 +83,9 @@ def _tuning(t):
 
 
 class Ensemble:
-    def __init__(self, solvers, dts, tuning=None, autotune=True):
+    def __init__(self, solvers, dts, tuning=None, autotune=True, tuning_cache=None):
+        """``tuning_cache``: path of the JSON tuning cache (None: the default
+        under the user's cache directory; False: none)."""
         if len(solvers) != len(dts) or not solvers:
             raise ValueError('one time step per solver')
         self.solvers = list(solvers)
@@ -71,9 +123,14 @@ class Ensemble:
         self.u2 = torch.empty_like(self.f)
         self.time_elapsed = 0.0
         self.steps_taken = 0
+        self.tuning_cache = DEFAULT_TUNING_CACHE if tuning_cache is None else tuning_cache
         key = self.signature()
         if tuning is None:
             tuning = _TUNED.get(key)
+        if tuning is None and self.tuning_cache:
+            tuning = load_tuning(self.tuning_cache, self._cache_key())
+            if tuning is not None:
+                _TUNED[key] = tuning
         if tuning is None:
             tuning = self.autotune() if autotune else DEFAULT_TUNING
         self._set_tuning(tuning)
@@ -91,6 +148,9 @@ class Ensemble:
         if s._layout == _lib.LAYOUT['q1-lines']:
             return f.reshape(s._P, s.N_q1)
         return f.reshape(s.N_q1, s._P).t()
+
+    def _cache_key(self):
+        return cache_key(self.signature(), torch.cuda.get_device_name(self.device))
 
     def signature(self):
         return tuple((s.N_q1, s.N_p1, s.N_p2, s.N_p3) for s in self.solvers) + \
@@ -209,5 +269,7 @@ class Ensemble:
         self.f = saved_f
         self.steps_taken = 0
         _TUNED[self.signature()] = best
+        if self.tuning_cache:
+            store_tuning(self.tuning_cache, self._cache_key(), best)
         self.tune_time_per_step = best_t / steps
         return best

@@ -45,3 +45,27 @@ def test_workspace_grows_with_tasks_and_checks_input():
     two_d[0].grid.n_q2 = 8
     assert L.boltx_ensemble_layout(1, two_d, None) == -1
     assert b"N_q2 = 1" in L.boltx_last_error()
+
+
+def test_tuning_cache_file(tmp_path):
+    """The JSON tuning cache: entries keyed by GPU model and signature,
+    added without losing others, and an unreadable or malformed file (or
+    entry) reads as a miss."""
+    from bolt_amd import ensemble as E
+    path = str(tmp_path / "sub" / "tuning.json")
+    k1 = E.cache_key(((32, 8, 8, 1), (64, 6, 6, 1), 1, 1, 3), "AMD Instinct MI355X")
+    k2 = E.cache_key(((32, 8, 8, 1),), "AMD Instinct MI355X")
+    assert E.load_tuning(path, k1) is None
+    assert E.store_tuning(path, k1, (4, 1024, 8, 65536))
+    assert E.store_tuning(path, k2, (2, 256, 2, 16384))
+    assert E.load_tuning(path, k1) == (4, 1024, 8, 65536)
+    assert E.load_tuning(path, k2) == (2, 256, 2, 16384)
+    assert E.load_tuning(path, E.cache_key(((32, 8, 8, 1),), "other GPU")) is None
+    with open(path, "w") as fh:
+        fh.write('{"%s": [1, 2]}' % k1.replace('"', '\\"'))
+    assert E.load_tuning(path, k1) is None
+    with open(path, "w") as fh:
+        fh.write("not json")
+    assert E.load_tuning(path, k1) is None
+    assert E.store_tuning(path, k1, (1, 256, 2, 4096))
+    assert E.load_tuning(path, k1) == (1, 256, 2, 4096)

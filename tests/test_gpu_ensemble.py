@@ -141,3 +141,26 @@ def test_graph_replays_match_eager_steps(gpu):
         torch.cuda.synchronize()
         assert torch.equal(eager.f, graph.f)
     graph.check_divergence()
+
+
+def test_tuning_cache_is_read_instead_of_tuning_again(gpu, tmp_path, monkeypatch):
+    """The autotuned shape is stored in the JSON cache under the ensemble's
+    signature and the GPU model; a later ensemble with the same signature
+    (its in-process cache cleared, as in a new process) takes it from the
+    file and does not tune."""
+    import os
+    from bolt_amd import ensemble as E
+    cases = _members("fp32-bulk", SIZES[:4])
+    path = str(tmp_path / "tuning.json")
+    E._TUNED.clear()
+    a = E.Ensemble([hip_solver(c) for c in cases], [c.dt for c in cases], tuning_cache=path)
+    assert os.path.exists(path)
+    assert E.load_tuning(path, a._cache_key()) == a.tuning
+    E._TUNED.clear()
+
+    def no_tuning(self, *args, **kw):
+        raise AssertionError("tuned again despite the cache")
+    monkeypatch.setattr(E.Ensemble, "autotune", no_tuning)
+    b = E.Ensemble([hip_solver(c) for c in cases], [c.dt for c in cases], tuning_cache=path)
+    assert b.tuning == a.tuning
+    E._TUNED.clear()
