@@ -20,6 +20,7 @@
 #include "../../include/boltx.h"
 #include "fvm_stage.h"
 #include "qcell.h"
+#include "qsweep.h"
 #include "ql_launch.h"
 #include "qline_moments.h"
 #include "adjoint.h"
@@ -492,9 +493,24 @@ static bool qc_applies(const boltx_grid* g) {
 
 static size_t qc_workspace_bytes(int nslab) { return 2 * (size_t)nslab * 64 * sizeof(int); }
 
+// LDS bytes of qsweep_kernel: tables, the padded p-block and its accumulator.
+static size_t qsweep_lds(const StageParams& sp) {
+  const size_t pp = (size_t)(sp.n_p1 + 1) * sp.n_p2 * sp.n_p3;
+  return sizeof(float) * ((size_t)(sp.n_p1 + sp.n_p2 + sp.n_p3) + 2 * pp);
+}
+
 template <typename T, int RQ, int RP, bool PADV>
 static int launch_qc_t(const QCParams& qp, hipStream_t s) {
   const StageParams& sp = qp.sp;
+  if constexpr (sizeof(T) == 4 && PADV && RP == RC_WENO5) {
+    // FP32 WENO5 p-advection: the p-line sweeps, when the padded block and
+    // the accumulator fit one CU's LDS (one 16-wave workgroup per CU)
+    const size_t lq = qsweep_lds(sp);
+    if (lq + 1024 <= 160 * 1024) {
+      hipLaunchKernelGGL((qsweep_kernel<RQ, 1024>), dim3(sp.Q), dim3(1024), lq, s, qp);
+      return launch_status("qsweep_kernel");
+    }
+  }
   size_t lds = sizeof(T) * (size_t)(sp.n_p1 + sp.n_p2 + sp.n_p3);
   if (PADV) lds += sizeof(T) * (size_t)sp.P;
   if (lds > 160 * 1024) return fail(BOLTX_EUNSUPPORTED, "p-block of %zu bytes exceeds LDS", lds);

@@ -1,0 +1,319 @@
+// FP32-bulk RK stage with p-advection and WENO5 in p, one workgroup per
+// q-cell, with the p-fluxes computed by sweeps along p-lines.
+//
+// qcell_kernel evaluates the three p-flux differences of every phase-space
+// cell on its own: per axis, two face states from four WENO indicators,
+// gathered from a seven-cell window with periodic index arithmetic.  Here
+// each p-axis is swept instead: a thread marches along a segment of one
+// p-line of the q-cell's block (in LDS), computing each face state once and
+// each cell's indicator once, and the flux difference of every cell is added
+// into an LDS accumulator.  The q-fluxes, BGK source, RK combination and
+// moments then run over the sign-grouped p order as in qcell_kernel, taking
+// the p-part from the accumulator.
+//
+// C_p along its own axis does not depend on that axis's velocity
+// (advection_terms.py:147-188), so C is constant along a p-line and one sign
+// per line selects the upwind state: the right face of cell j (window
+// j-2..j+2) when C > 0, else the left face of cell j+1 (window j-1..j+3).
+// The reference's left-face formula on a window z0..z4 equals its right-face
+// formula on the reversed window z4..z0 with the three indicators reversed
+// (weno5.py:73-96; the indicators themselves keep their forward formula,
+// weno5.py:58-71), so both cases are one formula on selected operands and
+// lines of either sign share a wave without divergence.  LLF equals upwind
+// for this linear flux (C is the same on both sides of every face).
+//
+// LDS: the q-cell's block with each p1-line padded to n_p1 + 1 floats (an odd
+// pitch for n_p1 = 24, so a wave's 64 lines hit 64 distinct banks), the
+// accumulator in the same padded layout, and the Maxwellian factor tables.
+//
+// Reference: df_dt_fvm.py:251-323 (p fluxes), weno5.py:57-98,
+// riemann.py:46-49; the rest as qcell_kernel.
+#pragma once
+#include "qcell.h"
+
+namespace bx {
+
+constexpr int QS_SEG = 8;  // cells per sweep segment
+
+// x / d and x % d for 0 <= x < 2^24 and 0 < d < 2^24 through the FP32
+// reciprocal rcp_d = 1/d (the quotient estimate is off by at most one and
+// corrected), instead of an integer division sequence.
+__device__ __forceinline__ int qs_divmod(int x, int d, float rcp_d, int& rem) {
+  int qt = int(float(x) * rcp_d);
+  int r = x - qt * d;
+  if (r < 0) {
+    --qt;
+    r += d;
+  } else if (r >= d) {
+    ++qt;
+    r -= d;
+  }
+  rem = r;
+  return qt;
+}
+
+// The reference's WENO5 indicator (sum-of-squares form of weno5.py:58-71) as
+// kappa = 1 / (16 beta^2), so a weight d/16/beta^2 is d * kappa.
+__device__ __forceinline__ float qs_kappa(float a, float b, float c) {
+  const float s1 = __builtin_fmaf(-2.0f, b, a) + c;
+  const float s2 = __builtin_fmaf(3.0f, c, __builtin_fmaf(-4.0f, b, a));
+  const float e = 1e-17f * (((1.0f + aabs(a)) + aabs(b)) + aabs(c));
+  const float bt = __builtin_fmaf((13.0f / 12.0f) * s1, s1, __builtin_fmaf(0.25f * s2, s2, e));
+  return __builtin_amdgcn_rcpf(16.0f * bt * bt);
+}
+
+// Right-face WENO5 state of the centre of y0..y4 with indicator quantities
+// k1, k2, k3 of the three substencils (weights 1, 10, 5 in units of 1/16).
+__device__ __forceinline__ float qs_face(float y0, float y1, float y2, float y3, float y4,
+                                         float k1, float k2, float k3) {
+  const float u1 = __builtin_fmaf(1.875f, y2, __builtin_fmaf(-1.25f, y1, 0.375f * y0));
+  const float u2 = __builtin_fmaf(0.375f, y3, __builtin_fmaf(0.75f, y2, -0.125f * y1));
+  const float u3 = __builtin_fmaf(-0.125f, y4, __builtin_fmaf(0.75f, y3, 0.375f * y2));
+  const float w2 = 10.0f * k2, w3 = 5.0f * k3;
+  const float num = __builtin_fmaf(w3, u3, __builtin_fmaf(w2, u2, k1 * u1));
+  const float den = (k1 + w2) + w3;
+  return num * __builtin_amdgcn_rcpf(den);
+}
+
+// Upwind state of face j+1/2 from the window y[k] = f[j-3+k], k = 0..6, and
+// the indicator quantities kp[k] of cells j-1..j+2: the right face of cell j
+// (y[1..5], kp[0..2]) when pos, else the left face of cell j+1 as the
+// right-face formula on the reversed window (y[6..2], kp[3..1]); operands are
+// selected per lane, then one face is evaluated.
+__device__ __forceinline__ float qs_select_state(const float y[7], const float kp[4], bool pos) {
+  const float a0 = pos ? y[1] : y[6], a1 = pos ? y[2] : y[5], a2 = pos ? y[3] : y[4];
+  const float a3 = pos ? y[4] : y[3], a4 = pos ? y[5] : y[2];
+  const float k1 = pos ? kp[0] : kp[3], k2 = pos ? kp[1] : kp[2], k3 = pos ? kp[2] : kp[1];
+  return qs_face(a0, a1, a2, a3, a4, k1, k2, k3);
+}
+
+// Sweep of one p-axis: every line (the other two p-indices fixed) is cut into
+// segments of QS_SEG cells; thread t takes task t, t+NT, ... with consecutive
+// tasks on consecutive lines of one segment index.  acc[cell] = (or +=)
+// (F(j+1/2) - F(j-1/2)) * inv_dp.
+template <int AX, int NT>
+__device__ inline void qs_sweep(const StageParams& sp, const QCell<float>& c, const float* blk,
+                                float* acc, int pitch1) {
+  const int n = AX == 0 ? sp.n_p1 : AX == 1 ? sp.n_p2 : sp.n_p3;
+  if (n <= 1) return;
+  const int stride = AX == 0 ? 1 : AX == 1 ? pitch1 : pitch1 * sp.n_p2;
+  // the two other axes: (ia, na) fastest across lines, then (ib, nb)
+  const int na = AX == 0 ? sp.n_p2 : sp.n_p1;
+  const int nb = AX == 2 ? sp.n_p2 : sp.n_p3;
+  const int nlines = na * nb;
+  const int nseg = (n + QS_SEG - 1) / QS_SEG;
+  const float rcp_lines = 1.0f / float(nlines), rcp_na = 1.0f / float(na);
+  const double em = sp.charge / sp.mass;
+  const float inv_dp = float(AX == 0 ? sp.inv_dp1 : AX == 1 ? sp.inv_dp2 : sp.inv_dp3);
+  for (int t = threadIdx.x; t < nlines * nseg; t += NT) {
+    int line, ia;
+    const int seg = qs_divmod(t, nlines, rcp_lines, line);
+    const int ib = qs_divmod(line, na, rcp_na, ia);
+    // velocities of the fixed indices and the line's C (cell-centred fields)
+    int base;
+    double C;
+    if (AX == 0) {
+      const double v2 = centre(sp.p2s, ia, sp.dp2), v3 = centre(sp.p3s, ib, sp.dp3);
+      C = em * (c.E[0] + v2 * c.B[0][2] - v3 * c.B[0][1]);
+      base = pitch1 * (ia + sp.n_p2 * ib);
+    } else if (AX == 1) {
+      const double v1 = centre(sp.p1s, ia, sp.dp1), v3 = centre(sp.p3s, ib, sp.dp3);
+      C = em * (c.E[1] + v3 * c.B[1][0] - v1 * c.B[1][2]);
+      base = ia + pitch1 * sp.n_p2 * ib;
+    } else {
+      const double v1 = centre(sp.p1s, ia, sp.dp1), v2 = centre(sp.p2s, ib, sp.dp2);
+      C = em * (c.E[2] + v1 * c.B[2][1] - v2 * c.B[2][0]);
+      base = ia + pitch1 * ib;
+    }
+    const float Cf = float(C);
+    const bool pos = Cf > 0.0f;
+    const int j0 = seg * QS_SEG;
+    const int len = n - j0 < QS_SEG ? n - j0 : QS_SEG;
+    // window y[k] = cell j0-4+k, k = 0 .. QS_SEG+6, periodic along the line
+    // (a short last segment loads cells it does not use); the line offset
+    // advances by stride and wraps to 0 at the line's end
+    float y[QS_SEG + 7];
+    {
+      int j = j0 - 4;
+      j = n >= 4 ? (j < 0 ? j + n : j) : ((j % n) + n) % n;
+      int off = j * stride;
+      const int end = n * stride;
+#pragma unroll
+      for (int k = 0; k < QS_SEG + 7; ++k) {
+        y[k] = blk[base + off];
+        off += stride;
+        off = off == end ? 0 : off;
+      }
+    }
+    // indicator quantity kp[k] of cell j0-3+k (window y[k..k+2]), k = 0 .. len+4
+    float kp[QS_SEG + 5];
+#pragma unroll
+    for (int k = 0; k < QS_SEG + 5; ++k) kp[k] = qs_kappa(y[k], y[k + 1], y[k + 2]);
+    // flux through the lower face of cell j0 (face j+1/2 of j = j0-1: window
+    // cells j0-4 .. j0+2, indicators of cells j0-2 .. j0+1)
+    float wl[7], kl[4];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) wl[k] = y[k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) kl[k] = kp[k + 1];
+    float Flo = Cf * qs_select_state(wl, kl, pos);
+#pragma unroll
+    for (int s = 0; s < QS_SEG; ++s) {
+      if (s < len) {
+        // face (j0+s)+1/2: window cells j0+s-3 .. j0+s+3 = y[s+1 .. s+7],
+        // indicators of cells j0+s-1 .. j0+s+2 = kp[s+2 .. s+5]
+        float w[7], kk[4];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) w[k] = y[s + 1 + k];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) kk[k] = kp[s + 2 + k];
+        const float Fhi = Cf * qs_select_state(w, kk, pos);
+        const int at = base + (j0 + s) * stride;
+        const float dflux = (Fhi - Flo) * inv_dp;
+        if (AX == 0) acc[at] = dflux;
+        else acc[at] = acc[at] + dflux;
+        Flo = Fhi;
+      }
+    }
+  }
+}
+
+template <int RQ, int NT>
+__global__ __launch_bounds__(NT) void qsweep_kernel(QCParams qp) {
+  constexpr int NW = NT / 64;
+  extern __shared__ unsigned char smem[];
+  __shared__ double red[5][NW];
+  const StageParams& sp = qp.sp;
+  const int pitch1 = sp.n_p1 + 1;
+  const int PP = pitch1 * sp.n_p2 * sp.n_p3;
+  float* tab = reinterpret_cast<float*>(smem);
+  float* blk = tab + (sp.n_p1 + sp.n_p2 + sp.n_p3);
+  float* acc = blk + PP;
+  const int q = qc_order(sp, blockIdx.x, gridDim.x);
+  QCell<float> c;
+  qcell_neighbours(sp, q, c);
+  c.pref = 0.0;
+  c.e1 = c.e2 = c.e3 = tab;
+  if (sp.source) {
+    const BgkCell b = qcell_bgk(sp, q);
+    c.pref = maxwell_prefactor(b, sp.mass, sp.kB, sp.p_dim);
+    float* e1 = tab;
+    float* e2 = tab + sp.n_p1;
+    float* e3 = e2 + sp.n_p2;
+    for (int i = threadIdx.x; i < sp.n_p1; i += blockDim.x)
+      e1[i] = float(maxwell_factor(centre(sp.p1s, i, sp.dp1), b.u1, b.T, sp.mass, sp.kB));
+    for (int i = threadIdx.x; i < sp.n_p2; i += blockDim.x)
+      e2[i] = float(maxwell_factor(centre(sp.p2s, i, sp.dp2), b.u2, b.T, sp.mass, sp.kB));
+    for (int i = threadIdx.x; i < sp.n_p3; i += blockDim.x)
+      e3[i] = float(maxwell_factor(centre(sp.p3s, i, sp.dp3), b.u3, b.T, sp.mass, sp.kB));
+    c.e1 = e1;
+    c.e2 = e2;
+    c.e3 = e3;
+  }
+  const float* u = (const float*)sp.u;
+  const long P = sp.P;
+  const long base = (long)q * P;
+  qcell_fields(sp, c);
+  // the block into the padded layout (p1-lines of pitch n_p1 + 1); the
+  // accumulator is written by the p1 sweep, so it needs no clearing
+  for (int p = threadIdx.x; p < sp.P; p += NT) {
+    const int j1 = p % sp.n_p1, rest = p / sp.n_p1;
+    blk[j1 + pitch1 * rest] = u[base + p];
+  }
+  if (sp.n_p1 <= 1)
+    for (int i = threadIdx.x; i < PP; i += NT) acc[i] = 0.0f;
+  __syncthreads();
+  qs_sweep<0, NT>(sp, c, blk, acc, pitch1);
+  __syncthreads();
+  qs_sweep<1, NT>(sp, c, blk, acc, pitch1);
+  __syncthreads();
+  qs_sweep<2, NT>(sp, c, blk, acc, pitch1);
+  __syncthreads();
+
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const float* wv = (const float*)sp.w;
+  const float* unv = (const float*)sp.un;
+  float* outv = (float*)sp.out;
+  const bool use_un = sp.a != 0.0;
+  const bool w_is_u = sp.w == sp.u;
+  const float ta = float(sp.a), tb = float(sp.b), tcdt = float(sp.cdt);
+  const float tinv1 = float(sp.inv_dq1), tinv2 = float(sp.inv_dq2), tinvtau = float(sp.inv_tau);
+  constexpr int r = recon_reach(RQ) + 1;
+  long nb1[7], nb2[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    nb1[k] = (long)c.nb1[k] * P;
+    nb2[k] = (long)c.nb2[k] * P;
+  }
+  double mac[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int s = wave; s < qp.nslab; s += NW) {
+    const int p_lane = qp.perm[s * 64 + lane];
+    const bool valid = p_lane >= 0;
+    const int p = valid ? p_lane : 0;
+    const int jp = valid ? qp.jpack[s * 64 + lane] : 0;
+    const int j1 = jp & 1023, j2 = (jp >> 10) & 1023, j3 = jp >> 20;
+    const double v1 = centre(sp.p1s, j1, sp.dp1);
+    const double v2 = centre(sp.p2s, j2, sp.dp2);
+    const double v3 = centre(sp.p3s, j3, sp.dp3);
+    const bool pos1 = __builtin_amdgcn_readfirstlane(int(v1 > 0.0)) != 0;
+    const bool pos2 = __builtin_amdgcn_readfirstlane(int(v2 > 0.0)) != 0;
+    const int at = j1 + pitch1 * (j2 + sp.n_p2 * j3);
+    const float f = blk[at];
+    float t1 = 0.0f, t2 = 0.0f;
+    if (sp.n_q1 > 1) {
+      float a[7];
+#pragma unroll
+      for (int k = 3 - r; k <= 3 + r; ++k) a[k] = k == 3 ? f : u[nb1[k] + p];
+      t1 = upwind_flux_diff<RQ, float>(a, float(v1), pos1);
+    }
+    if (sp.n_q2 > 1) {
+      float a[7];
+#pragma unroll
+      for (int k = 3 - r; k <= 3 + r; ++k) a[k] = k == 3 ? f : u[nb2[k] + p];
+      t2 = upwind_flux_diff<RQ, float>(a, float(v2), pos2);
+    }
+    float d = -t1 * tinv1 - t2 * tinv2;
+    if (sp.source) {
+      float fmb = float(c.pref) * c.e1[j1];
+      if (sp.p_dim >= 2) fmb = fmb * c.e2[j2];
+      if (sp.p_dim >= 3) fmb = fmb * c.e3[j3];
+      d = d + (fmb - f) * tinvtau;
+    }
+    d = d - acc[at];
+    const float wval = w_is_u ? f : wv[base + p];
+    const float tv = wval + d * tcdt;
+    const float x = use_un ? ta * unv[base + p] + tb * tv : (sp.b == 1.0 ? tv : tb * tv);
+    if (valid) {
+      outv[base + p] = x;
+      const double xd = double(x);
+      mac[0] += xd;
+      mac[1] += xd * v1;
+      mac[2] += xd * v2;
+      mac[3] += xd * v3;
+      mac[4] += (0.5 * xd) * ((v1 * v1 + v2 * v2) + v3 * v3);
+    }
+  }
+  if (sp.mom_out) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      double x = mac[k];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+      mac[k] = x;
+    }
+    if (lane == 0)
+      for (int k = 0; k < 5; ++k) red[k][wave] = mac[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double m[5];
+      for (int k = 0; k < 5; ++k) {
+        double x = red[k][0];
+        for (int w = 1; w < NW; ++w) x += red[k][w];
+        m[k] = x * sp.measure;
+      }
+      write_moment_block(sp.mom_out, sp.Q, q, m, sp.mass, sp.kB, sp.p_dim);
+    }
+  }
+}
+
+}  // namespace bx
