@@ -54,7 +54,8 @@ class Stage(ctypes.Structure):
                 ("a", ctypes.c_double), ("b", ctypes.c_double), ("cdt", ctypes.c_double),
                 ("mom_in", ctypes.c_void_p), ("mom_out", ctypes.c_void_p),
                 ("fields_c_p1", ctypes.c_void_p), ("fields_c_p2", ctypes.c_void_p),
-                ("fields_c_p3", ctypes.c_void_p), ("workspace", ctypes.c_void_p)]
+                ("fields_c_p3", ctypes.c_void_p), ("workspace", ctypes.c_void_p),
+                ("currents_out", ctypes.c_void_p)]
 
 
 class Coeff(ctypes.Structure):

@@ -710,12 +710,25 @@ static int check_stage(const boltx_grid* grid, const boltx_physics* phys, const 
   if (phys->source_enabled && !st->mom_in) return fail(BOLTX_EINVAL, "mom_in is NULL");
   if (phys->p_advection && (!st->fields_c_p1 || !st->fields_c_p2 || !st->fields_c_p3))
     return fail(BOLTX_EINVAL, "fields are NULL with p_advection on");
+  if (st->currents_out && grid->layout != BOLTX_LAYOUT_REFERENCE)
+    return fail(BOLTX_EUNSUPPORTED, "currents_out needs the reference layout");
   const int need = recon_reach(phys->recon_q) + 1;
   if ((grid->n_q1 > 1 && grid->n_q1 < need) || (grid->n_q2 > 1 && grid->n_q2 < need))
     return fail(BOLTX_EINVAL, "q-grid smaller than the stencil reach");
   if (ghost_rows(grid) > 0 && grid->n_ghost < need)
     return fail(BOLTX_EINVAL, "n_ghost = %d is below the stencil reach %d", grid->n_ghost, need);
   return BOLTX_OK;
+}
+
+template <typename T>
+static int launch_currents(const StageParams& sp, int rq, double* J, hipStream_t s);
+
+// The currents of sp.u (boltx_currents) on the stream, for the stage paths
+// whose kernel does not form them itself.
+static int currents_of(const StageParams& sp, const boltx_physics* phys, double* J,
+                       hipStream_t s) {
+  if (phys->precision == BOLTX_FP64) return launch_currents<double>(sp, phys->recon_q, J, s);
+  return launch_currents<float>(sp, phys->recon_q, J, s);
 }
 
 // One stage with the given workspace (the caller's, or the scratch array).
@@ -743,6 +756,7 @@ static int fvm_stage_impl(const boltx_grid* grid, const boltx_physics* phys,
   sp.fp2 = st->fields_c_p2;
   sp.fp3 = st->fields_c_p3;
   const bool padv = phys->p_advection != 0;
+  int rc;
   if (grid->layout == BOLTX_LAYOUT_Q1_LINES) {
     if (padv) return fail(BOLTX_EUNSUPPORTED, "p-advection needs the reference layout");
     if (!workspace) return fail(BOLTX_EINVAL, "the q1-line layout needs the workspace");
@@ -750,7 +764,9 @@ static int fvm_stage_impl(const boltx_grid* grid, const boltx_physics* phys,
     return ql_stage<float>(grid, phys, st, workspace, sp, s);
   }
   if (workspace && qc_applies(grid)) {
+    // the slab kernels accumulate the currents of u with the moments
     QCParams qp;
+    sp.Jout = st->currents_out;
     qp.sp = sp;
     qp.nslab = qc_nslab(grid);
     qp.perm = (const int*)workspace;
@@ -759,6 +775,7 @@ static int fvm_stage_impl(const boltx_grid* grid, const boltx_physics* phys,
       return launch_qc<double>(qp, phys->recon_q, phys->recon_p, padv, s);
     return launch_qc<float>(qp, phys->recon_q, phys->recon_p, padv, s);
   }
+  if (st->currents_out && (rc = currents_of(sp, phys, st->currents_out, s))) return rc;
   if (phys->precision == BOLTX_FP64)
     return launch_stage_rq<double>(sp, phys->recon_q, phys->recon_p, padv, s);
   return launch_stage_rq<float>(sp, phys->recon_q, phys->recon_p, padv, s);
@@ -794,6 +811,8 @@ extern "C" int boltx_fvm_stage_coeffs(const boltx_grid* grid, const boltx_physic
     return fail(BOLTX_EINVAL, "c_p needs p_advection");
   if (co->source.ptr && co->tau.ptr)
     return fail(BOLTX_EINVAL, "tau applies to the built-in source, not to a caller's source");
+  if (st && st->currents_out && (co->c_q1.ptr || co->c_q2.ptr))
+    return fail(BOLTX_EINVAL, "currents_out takes its face states from the built-in C_q");
   // with the caller's source no moment block of u is read, and with the
   // caller's C_p no fields: check the stage as if both were there
   boltx_stage chk = *st;
@@ -835,6 +854,7 @@ extern "C" int boltx_fvm_stage_coeffs(const boltx_grid* grid, const boltx_physic
   if (co->source.ptr) sp.mom_in = nullptr;
   hipStream_t s = (hipStream_t)stream;
   const bool padv = phys->p_advection != 0;
+  if (st->currents_out && (rc = currents_of(sp, phys, st->currents_out, s))) return rc;
   if (phys->precision == BOLTX_FP64)
     return launch_stage_rq<double>(sp, phys->recon_q, phys->recon_p, padv, s);
   return launch_stage_rq<float>(sp, phys->recon_q, phys->recon_p, padv, s);

@@ -50,6 +50,9 @@ struct StageParams {
   // at bottom q2-faces, C_p1..3 at the left / bottom / back p-faces, an
   // additive source replacing BGK, the BGK collision time per cell
   Coeff cq1, cq2, cp1, cp2, cp3, src, tau;
+  // [3][Q] currents of u (charge * mom_v1(f_left), mom_v2(f_bot), mom_v3(f)),
+  // accumulated by the slab kernels, or NULL
+  double* Jout;
 };
 
 // Per-q-cell state the stage needs besides f: the stencil neighbours of the

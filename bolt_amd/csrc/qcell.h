@@ -54,36 +54,43 @@ __host__ __device__ inline float weno_face_betas(float y0, float y1, float y2, f
 }
 
 // Upwind flux difference C*(s(i+1/2) - s(i-1/2)) of cell i from a[k] =
-// f[i-3+k] with the upwind direction ``pos`` (C > 0) uniform over the wave.
+// f[i-3+k] with the upwind direction ``pos`` (C > 0) uniform over the wave;
+// the upwind state of the lower face s(i-1/2) (f_left / f_bot of
+// df_dt_fvm.py:79-80) is stored in ``lower``.
 template <int R, typename T>
-__host__ __device__ inline T upwind_flux_diff(const T a[7], T C, bool pos) {
+__host__ __device__ inline T upwind_flux_diff(const T a[7], T C, bool pos, T& lower) {
+  T sL, sR;
   if constexpr (R == RC_WENO5) {
     if (pos) {
       // right faces of cells i-1 and i: windows a0..a4 and a1..a5
       const T b0 = weno_beta<T>(a[0], a[1], a[2]), b1 = weno_beta<T>(a[1], a[2], a[3]);
       const T b2 = weno_beta<T>(a[2], a[3], a[4]), b3 = weno_beta<T>(a[3], a[4], a[5]);
-      const T sL = weno_face_betas(a[0], a[1], a[2], a[3], a[4], b0, b1, b2, true);
-      const T sR = weno_face_betas(a[1], a[2], a[3], a[4], a[5], b1, b2, b3, true);
-      return C * sR - C * sL;
+      sL = weno_face_betas(a[0], a[1], a[2], a[3], a[4], b0, b1, b2, true);
+      sR = weno_face_betas(a[1], a[2], a[3], a[4], a[5], b1, b2, b3, true);
     } else {
       // left faces of cells i and i+1: windows a1..a5 and a2..a6
       const T b1 = weno_beta<T>(a[1], a[2], a[3]), b2 = weno_beta<T>(a[2], a[3], a[4]);
       const T b3 = weno_beta<T>(a[3], a[4], a[5]), b4 = weno_beta<T>(a[4], a[5], a[6]);
-      const T sL = weno_face_betas(a[1], a[2], a[3], a[4], a[5], b1, b2, b3, false);
-      const T sR = weno_face_betas(a[2], a[3], a[4], a[5], a[6], b2, b3, b4, false);
-      return C * sR - C * sL;
+      sL = weno_face_betas(a[1], a[2], a[3], a[4], a[5], b1, b2, b3, false);
+      sR = weno_face_betas(a[2], a[3], a[4], a[5], a[6], b2, b3, b4, false);
     }
   } else {
     if (pos) {
-      const T sL = face_value<R>(a[0], a[1], a[2], a[3], a[4], true);
-      const T sR = face_value<R>(a[1], a[2], a[3], a[4], a[5], true);
-      return C * sR - C * sL;
+      sL = face_value<R>(a[0], a[1], a[2], a[3], a[4], true);
+      sR = face_value<R>(a[1], a[2], a[3], a[4], a[5], true);
     } else {
-      const T sL = face_value<R>(a[1], a[2], a[3], a[4], a[5], false);
-      const T sR = face_value<R>(a[2], a[3], a[4], a[5], a[6], false);
-      return C * sR - C * sL;
+      sL = face_value<R>(a[1], a[2], a[3], a[4], a[5], false);
+      sR = face_value<R>(a[2], a[3], a[4], a[5], a[6], false);
     }
   }
+  lower = sL;
+  return C * sR - C * sL;
+}
+
+template <int R, typename T>
+__host__ __device__ inline T upwind_flux_diff(const T a[7], T C, bool pos) {
+  T lower;
+  return upwind_flux_diff<R, T>(a, C, pos, lower);
 }
 
 // Flux difference along a p axis of the q-cell's p-block ``blk`` (stride
@@ -133,11 +140,47 @@ __device__ inline int qc_order(const StageParams& sp, int b, int nb) {
   return (l / rows) + sp.n_q1 * (x * rows + l % rows);
 }
 
+// End of a slab kernel: the workgroup sums (butterfly within waves, waves in
+// order) of the five moments of the output and, if requested, the three
+// currents of the input, written as the q-cell's moment block and J.
+template <int NW>
+__device__ inline void qc_finish(const StageParams& sp, int q, double (&acc)[5], double (&jac)[3],
+                                 double (*red)[NW], int wave, int lane) {
+  const bool mom = sp.mom_out != nullptr, cur = sp.Jout != nullptr;
+  if (!mom && !cur) return;
+  double v[8];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) v[k] = acc[k];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) v[5 + k] = jac[k];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    double x = v[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    v[k] = x;
+  }
+  if (lane == 0)
+    for (int k = 0; k < 8; ++k) red[k][wave] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m[8];
+    for (int k = 0; k < 8; ++k) {
+      double x = red[k][0];
+      for (int w = 1; w < NW; ++w) x += red[k][w];
+      m[k] = x * sp.measure;
+    }
+    if (mom) write_moment_block(sp.mom_out, sp.Q, q, m, sp.mass, sp.kB, sp.p_dim);
+    if (cur)
+      for (int k = 0; k < 3; ++k) sp.Jout[k * sp.Q + q] = sp.charge * m[5 + k];
+  }
+}
+
 template <typename T, int RQ, int RP, bool PADV, int NT>
 __global__ __launch_bounds__(NT) void qcell_kernel(QCParams qp) {
   constexpr int NW = NT / 64;
   extern __shared__ unsigned char smem[];
-  __shared__ double red[5][NW];
+  __shared__ double red[8][NW];
   const StageParams& sp = qp.sp;
   T* tab = reinterpret_cast<T*>(smem);
   T* blk = tab + (sp.n_p1 + sp.n_p2 + sp.n_p3);
@@ -187,6 +230,7 @@ __global__ __launch_bounds__(NT) void qcell_kernel(QCParams qp) {
     nb2[k] = (long)c.nb2[k] * P;
   }
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double jac[3] = {0.0, 0.0, 0.0};
   for (int s = wave; s < qp.nslab; s += NW) {
     const int p_lane = qp.perm[s * 64 + lane];
     const bool valid = p_lane >= 0;
@@ -199,18 +243,23 @@ __global__ __launch_bounds__(NT) void qcell_kernel(QCParams qp) {
     const bool pos1 = __builtin_amdgcn_readfirstlane(int(v1 > 0.0)) != 0;
     const bool pos2 = __builtin_amdgcn_readfirstlane(int(v2 > 0.0)) != 0;
     const T f = u[base + p];
-    T t1 = T(0), t2 = T(0);
+    T t1 = T(0), t2 = T(0), s1 = f, s2 = f;
     if (sp.n_q1 > 1) {
       T a[7];
 #pragma unroll
       for (int k = 3 - r; k <= 3 + r; ++k) a[k] = k == 3 ? f : u[nb1[k] + p];
-      t1 = upwind_flux_diff<RQ, T>(a, T(v1), pos1);
+      t1 = upwind_flux_diff<RQ, T>(a, T(v1), pos1, s1);
     }
     if (sp.n_q2 > 1) {
       T a[7];
 #pragma unroll
       for (int k = 3 - r; k <= 3 + r; ++k) a[k] = k == 3 ? f : u[nb2[k] + p];
-      t2 = upwind_flux_diff<RQ, T>(a, T(v2), pos2);
+      t2 = upwind_flux_diff<RQ, T>(a, T(v2), pos2, s2);
+    }
+    if (valid && sp.Jout) {
+      jac[0] += double(s1) * v1;
+      jac[1] += double(s2) * v2;
+      jac[2] += double(f) * v3;
     }
     T d = -t1 * tinv1 - t2 * tinv2;
     if (sp.source) {
@@ -250,27 +299,7 @@ __global__ __launch_bounds__(NT) void qcell_kernel(QCParams qp) {
       acc[4] += (0.5 * xd) * ((v1 * v1 + v2 * v2) + v3 * v3);
     }
   }
-  if (sp.mom_out) {
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-      double x = acc[k];
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-      acc[k] = x;
-    }
-    if (lane == 0)
-      for (int k = 0; k < 5; ++k) red[k][wave] = acc[k];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double m[5];
-      for (int k = 0; k < 5; ++k) {
-        double x = red[k][0];
-        for (int w = 1; w < NW; ++w) x += red[k][w];
-        m[k] = x * sp.measure;
-      }
-      write_moment_block(sp.mom_out, sp.Q, q, m, sp.mass, sp.kB, sp.p_dim);
-    }
-  }
+  qc_finish<NW>(sp, q, acc, jac, red, wave, lane);
 }
 
 }  // namespace bx

@@ -182,7 +182,7 @@ template <int RQ, int NT>
 __global__ __launch_bounds__(NT) void qsweep_kernel(QCParams qp) {
   constexpr int NW = NT / 64;
   extern __shared__ unsigned char smem[];
-  __shared__ double red[5][NW];
+  __shared__ double red[8][NW];
   const StageParams& sp = qp.sp;
   const int pitch1 = sp.n_p1 + 1;
   const int PP = pitch1 * sp.n_p2 * sp.n_p3;
@@ -246,6 +246,7 @@ __global__ __launch_bounds__(NT) void qsweep_kernel(QCParams qp) {
     nb2[k] = (long)c.nb2[k] * P;
   }
   double mac[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double jac[3] = {0.0, 0.0, 0.0};
   for (int s = wave; s < qp.nslab; s += NW) {
     const int p_lane = qp.perm[s * 64 + lane];
     const bool valid = p_lane >= 0;
@@ -259,18 +260,23 @@ __global__ __launch_bounds__(NT) void qsweep_kernel(QCParams qp) {
     const bool pos2 = __builtin_amdgcn_readfirstlane(int(v2 > 0.0)) != 0;
     const int at = j1 + pitch1 * (j2 + sp.n_p2 * j3);
     const float f = blk[at];
-    float t1 = 0.0f, t2 = 0.0f;
+    float t1 = 0.0f, t2 = 0.0f, s1 = f, s2 = f;
     if (sp.n_q1 > 1) {
       float a[7];
 #pragma unroll
       for (int k = 3 - r; k <= 3 + r; ++k) a[k] = k == 3 ? f : u[nb1[k] + p];
-      t1 = upwind_flux_diff<RQ, float>(a, float(v1), pos1);
+      t1 = upwind_flux_diff<RQ, float>(a, float(v1), pos1, s1);
     }
     if (sp.n_q2 > 1) {
       float a[7];
 #pragma unroll
       for (int k = 3 - r; k <= 3 + r; ++k) a[k] = k == 3 ? f : u[nb2[k] + p];
-      t2 = upwind_flux_diff<RQ, float>(a, float(v2), pos2);
+      t2 = upwind_flux_diff<RQ, float>(a, float(v2), pos2, s2);
+    }
+    if (valid && sp.Jout) {
+      jac[0] += double(s1) * v1;
+      jac[1] += double(s2) * v2;
+      jac[2] += double(f) * v3;
     }
     float d = -t1 * tinv1 - t2 * tinv2;
     if (sp.source) {
@@ -293,27 +299,7 @@ __global__ __launch_bounds__(NT) void qsweep_kernel(QCParams qp) {
       mac[4] += (0.5 * xd) * ((v1 * v1 + v2 * v2) + v3 * v3);
     }
   }
-  if (sp.mom_out) {
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-      double x = mac[k];
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-      mac[k] = x;
-    }
-    if (lane == 0)
-      for (int k = 0; k < 5; ++k) red[k][wave] = mac[k];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double m[5];
-      for (int k = 0; k < 5; ++k) {
-        double x = red[k][0];
-        for (int w = 1; w < NW; ++w) x += red[k][w];
-        m[k] = x * sp.measure;
-      }
-      write_moment_block(sp.mom_out, sp.Q, q, m, sp.mass, sp.kB, sp.p_dim);
-    }
-  }
+  qc_finish<NW>(sp, q, mac, jac, red, wave, lane);
 }
 
 }  // namespace bx

@@ -611,7 +611,7 @@ class nonlinear_solver(object):
             fs.compute_electrostatic_fields(self._rho(self._stage_moms))
         elif fs.coupling == 'once-per-step':
             if stage == 0:
-                self._currents(us, 'J_n')
+                # J(u^n) comes out of this stage's kernels (_advance sums it)
                 fs.stage_fields = fs._cc_at_n
             elif stage == 1:
                 J_n = self._spc[0].bufs['J_n']
@@ -684,16 +684,17 @@ class nonlinear_solver(object):
                 c.tau = self._coeff(sp.tau_field, k, keep)
         return per, keep
 
-    def _stage_kernel(self, u, w, un, out, a, b, cdt, mom_in, mom_out, F, co=None):
+    def _stage_kernel(self, u, w, un, out, a, b, cdt, mom_in, mom_out, F, co=None, jout=None):
         """The current species' stage launch, out = a*un + b*(w + cdt*L(u)),
         with its ghost refresh (fvm_operator.py:22-33) and the moments of
         out; ``co`` (boltx_coeffs) carries the user's terms to the generic
-        kernel."""
+        kernel; ``jout`` ([3][Q] FP64) receives the currents of u."""
         self._fill_ghosts(u)
         if co is not None:
             st = _lib.Stage(_lib.ptr(u), _lib.ptr(w), _lib.ptr(un), _lib.ptr(out),
                             float(a), float(b), float(cdt), _lib.ptr(mom_in), _lib.ptr(mom_out),
-                            _lib.ptr(F[0]), _lib.ptr(F[1]), _lib.ptr(F[2]), None)
+                            _lib.ptr(F[0]), _lib.ptr(F[1]), _lib.ptr(F[2]), None,
+                            _lib.ptr(jout))
             _lib.check(self._L.boltx_fvm_stage_coeffs(
                 ctypes.byref(self._grid), ctypes.byref(self._phys), ctypes.byref(st),
                 ctypes.byref(co), _lib.stream_handle()))
@@ -704,7 +705,8 @@ class nonlinear_solver(object):
         st = _lib.Stage(_lib.ptr(u), _lib.ptr(w), _lib.ptr(un), _lib.ptr(out),
                         float(a), float(b), float(cdt), _lib.ptr(mom_in),
                         None if split else _lib.ptr(mom_out),
-                        _lib.ptr(F[0]), _lib.ptr(F[1]), _lib.ptr(F[2]), _lib.ptr(self._ws))
+                        _lib.ptr(F[0]), _lib.ptr(F[1]), _lib.ptr(F[2]), _lib.ptr(self._ws),
+                        _lib.ptr(jout))
         ev = self._stage_events
         if ev is not None:
             e0 = torch.cuda.Event(enable_timing=True)
@@ -762,9 +764,18 @@ class nonlinear_solver(object):
             co = [None] * len(self._spc)
             if self._general:
                 co, keep = self._callback_terms([x[0] for x in args])
+            # SSP-RK3 with FDTD: stage 1's kernels also give J(u^n) (df_dt_fvm.py:214-224)
+            fuse_j = stage == 0 and self.fields_solver is not None and \
+                self.fields_solver.coupling == 'once-per-step'
             for k in self._species_loop():
                 uu, ww, nn, oo, mmi, mmo = args[k]
-                self._stage_kernel(uu, ww, nn, oo, a, b, cdt, mmi, mmo, F, co[k])
+                jk = self._buf('J_stage', F64, (3, self._Q)) if fuse_j else None
+                self._stage_kernel(uu, ww, nn, oo, a, b, cdt, mmi, mmo, F, co[k], jk)
+            if fuse_j:
+                J = self._spc[0].bufs.setdefault('J_n', torch.empty_like(jk))
+                J.copy_(self._spc[0].bufs['J_stage'])
+                for sp in self._spc[1:]:
+                    J += sp.bufs['J_stage']
         for _ in self._species_loop():
             _lib.check(self._L.boltx_check_finite(ctypes.byref(self._grid), _lib.ptr(self._mom),
                                                   _lib.ptr(self._flag), _lib.stream_handle()))

@@ -109,6 +109,9 @@ typedef struct boltx_stage {
   const double *fields_c_p3; /* [6][Q] fields seen by C_p3 */
   void *workspace;  /* boltx_workspace_size() bytes set up by boltx_workspace_init(),
                        or NULL; stages sharing a workspace must be stream-ordered */
+  double *currents_out; /* [3][Q] FP64 or NULL: the currents of u as boltx_currents
+                           computes them (reference layout only), written by the
+                           stage itself where its kernel forms the face states */
 } boltx_stage;
 
 /* One Runge-Kutta stage of the FVM step: out = a*un + b*(w + cdt*L(u)) where
