@@ -137,8 +137,9 @@ def _pmc_rows(path, counter):
 
 
 def traffic_from_pmc(key, fetch_csv, write_csv, kernel, algorithmic):
-    """HBM bytes per launch of ``kernel`` from one FETCH_SIZE run and one
-    WRITE_SIZE run of ``bench.py --probe`` (KiB per dispatch).  Each counter
+    """HBM bytes per launch of ``kernel`` (names separated by '|') from one
+    FETCH_SIZE run and one WRITE_SIZE run of ``bench.py --probe`` (KiB per
+    dispatch).  Each counter
     is scaled by what it reads for the probe's first 256 MiB device copy
     (read 256 MiB, write 256 MiB) in the same run.  ``algorithmic`` are the
     algorithmic bytes of each launch, in order.  Updates
@@ -148,7 +149,7 @@ def traffic_from_pmc(key, fetch_csv, write_csv, kernel, algorithmic):
     for name, path in (("fetch", fetch_csv), ("write", write_csv)):
         rows = _pmc_rows(path, name.upper() + "_SIZE")
         cal = [v for k, v in rows if "copy" in k.lower()]
-        launches = [v for k, v in rows if kernel in k]
+        launches = [v for k, v in rows if any(n in k for n in kernel.split("|"))]
         if not cal or not launches:
             raise SystemExit("no copy or %s dispatches in %s" % (kernel, path))
         out[name + "_scale"] = mib256 / cal[0]
@@ -458,7 +459,7 @@ def main():
     args = ap.parse_args()
     if args.traffic:
         w = args.probe_of
-        kernel = "qcell_kernel" if w.startswith("em-") else "qline_kernel"
+        kernel = "qcell_kernel|qsweep_kernel" if w.startswith("em-") else "qline_kernel"
         key = "em_" + w[3:] if w.startswith("em-") else w
         e = traffic_from_pmc(key, args.traffic[0], args.traffic[1], kernel, probe_algorithmic(w))
         print(json.dumps({key: e}, indent=1), flush=True)

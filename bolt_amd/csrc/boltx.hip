@@ -206,6 +206,17 @@ __global__ __launch_bounds__(64) void slab_perm_kernel(int* perm, int* jpack, in
 // J1 = e * sum(f_left v1) dV, J2 = e * sum(f_bot v2) dV, J3 = e * sum(f v3) dV
 // with f_left / f_bot the upwind states of the q1 / q2 left faces
 // (df_dt_fvm.py:79-80, 214-224).
+// Upwind state of the lower face of the centre of a[k] = f[i-3+k]
+// (riemann.py:46-49): the right face of cell i-1 when C > 0, else the left
+// face of cell i; one reconstruction from the window selected per lane.
+template <int RQ, typename T>
+__device__ inline T lower_face_state(const T a[7], bool pos) {
+  T w[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) w[k] = pos ? a[k] : a[k + 1];
+  return face_value<RQ>(w[0], w[1], w[2], w[3], w[4], pos);
+}
+
 template <typename T, int RQ>
 __global__ __launch_bounds__(kBlock) void currents_kernel(StageParams sp, double* J) {
   __shared__ double red[3][kWaves];
@@ -216,22 +227,27 @@ __global__ __launch_bounds__(kBlock) void currents_kernel(StageParams sp, double
   const long P = sp.P;
   constexpr int r = recon_reach(RQ) + 1;
   double acc[3] = {0.0, 0.0, 0.0};
+  const float rcp1 = 1.0f / float(sp.n_p1), rcp2 = 1.0f / float(sp.n_p2);
   for (int p = threadIdx.x; p < sp.P; p += blockDim.x) {
-    const int j1 = p % sp.n_p1, j23 = p / sp.n_p1;
+    int j1, j23, j2;
+    j23 = qs_divmod(p, sp.n_p1, rcp1, j1);
+    const int j3 = qs_divmod(j23, sp.n_p2, rcp2, j2);
     const double v1 = centre(sp.p1s, j1, sp.dp1);
-    const double v2 = centre(sp.p2s, j23 % sp.n_p2, sp.dp2);
-    const double v3 = centre(sp.p3s, j23 / sp.n_p2, sp.dp3);
+    const double v2 = centre(sp.p2s, j2, sp.dp2);
+    const double v3 = centre(sp.p3s, j3, sp.dp3);
     const T f = u[(long)q * P + p];
     T s1 = f, s2 = f;
     if (sp.n_q1 > 1) {
       T a[7];
+      for (int k = 0; k < 7; ++k) a[k] = f;
       for (int k = 3 - r; k <= 3 + r; ++k) a[k] = k == 3 ? f : u[(long)c.nb1[k] * P + p];
-      flux_diff<RQ, T>(a, T(v1), RS_UPWIND, &s1);
+      s1 = lower_face_state<RQ, T>(a, T(v1) > T(0));
     }
     if (sp.n_q2 > 1) {
       T a[7];
+      for (int k = 0; k < 7; ++k) a[k] = f;
       for (int k = 3 - r; k <= 3 + r; ++k) a[k] = k == 3 ? f : u[(long)c.nb2[k] * P + p];
-      flux_diff<RQ, T>(a, T(v2), RS_UPWIND, &s2);
+      s2 = lower_face_state<RQ, T>(a, T(v2) > T(0));
     }
     acc[0] += double(s1) * v1;
     acc[1] += double(s2) * v2;
@@ -397,6 +413,8 @@ int check_grid(const boltx_grid* g) {
   const long P = (long)g->n_p1 * g->n_p2 * g->n_p3;
   const long Q = (long)g->n_q1 * g->n_q2;
   if (P * Q > (1L << 40)) return fail(BOLTX_EINVAL, "grid too large");
+  // p-indices are split into (j1, j2, j3) through FP32 reciprocals
+  if (P >= (1L << 24)) return fail(BOLTX_EINVAL, "2^24 or more p-cells per q-cell");
   if (Q > 0x7fffffff) return fail(BOLTX_EINVAL, "too many q-cells");
   return BOLTX_OK;
 }
