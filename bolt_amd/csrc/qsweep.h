@@ -8,8 +8,9 @@
 // p-line of the q-cell's block (in LDS), computing each face state once and
 // each cell's indicator once, and the flux difference of every cell is added
 // into an LDS accumulator.  The q-fluxes, BGK source, RK combination and
-// moments then run over the sign-grouped p order as in qcell_kernel, taking
-// the p-part from the accumulator.
+// moments then run over the p-cells in natural order (a wave's loads and
+// stores are contiguous), with the upwind side of each q-face selected per
+// lane (qs_q_flux_diff), taking the p-part from the accumulator.
 //
 // C_p along its own axis does not depend on that axis's velocity
 // (advection_terms.py:147-188), so C is constant along a p-line and one sign
@@ -85,6 +86,41 @@ __device__ __forceinline__ float qs_select_state(const float y[7], const float k
   const float a3 = pos ? y[4] : y[3], a4 = pos ? y[5] : y[2];
   const float k1 = pos ? kp[0] : kp[3], k2 = pos ? kp[1] : kp[2], k3 = pos ? kp[2] : kp[1];
   return qs_face(a0, a1, a2, a3, a4, k1, k2, k3);
+}
+
+// Flux difference C*(s(i+1/2) - s(i-1/2)) along a q axis for the cell at the
+// centre of a[k] = f[i-3+k], with the upwind side chosen per lane: both faces
+// are right-face reconstructions of selected windows, the lower face of
+// window a[0..4] (C > 0: right face of cell i-1) or of the reversed a[5..1]
+// (C <= 0: left face of cell i), the upper face of a[1..5] or of the reversed
+// a[6..2].  PPM, minmod and piecewise-constant faces are mirror-symmetric
+// (ppm.py:83-98: of the two corrections at most one applies); for WENO5 the
+// forward indicators are assigned in reverse order (weno5.py:58-96).  The
+// lower face state is returned in ``lower`` (f_left / f_bot of
+// df_dt_fvm.py:79-80).
+template <int RQ>
+__device__ __forceinline__ float qs_q_flux_diff(const float a[7], float C, bool pos, float& lower) {
+  float L[5], U[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    L[k] = pos ? a[k] : a[5 - k];
+    U[k] = pos ? a[k + 1] : a[6 - k];
+  }
+  float sL, sU;
+  if constexpr (RQ == RC_WENO5) {
+    float b[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) b[k] = weno_beta<float>(a[k], a[k + 1], a[k + 2]);
+    sL = weno_combine(L[0], L[1], L[2], L[3], L[4], pos ? b[0] : b[3], pos ? b[1] : b[2],
+                      pos ? b[2] : b[1], true);
+    sU = weno_combine(U[0], U[1], U[2], U[3], U[4], pos ? b[1] : b[4], pos ? b[2] : b[3],
+                      pos ? b[3] : b[2], true);
+  } else {
+    sL = face_value<RQ>(L[0], L[1], L[2], L[3], L[4], true);
+    sU = face_value<RQ>(U[0], U[1], U[2], U[3], U[4], true);
+  }
+  lower = sL;
+  return C * sU - C * sL;
 }
 
 // Sweep of one p-axis: every line (the other two p-indices fixed) is cut into
@@ -247,33 +283,37 @@ __global__ __launch_bounds__(NT) void qsweep_kernel(QCParams qp) {
   }
   double mac[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   double jac[3] = {0.0, 0.0, 0.0};
-  for (int s = wave; s < qp.nslab; s += NW) {
-    const int p_lane = qp.perm[s * 64 + lane];
-    const bool valid = p_lane >= 0;
-    const int p = valid ? p_lane : 0;
-    const int jp = valid ? qp.jpack[s * 64 + lane] : 0;
-    const int j1 = jp & 1023, j2 = (jp >> 10) & 1023, j3 = jp >> 20;
+  const float rcp1 = 1.0f / float(sp.n_p1), rcp2 = 1.0f / float(sp.n_p2);
+  // natural p order: a wave's loads and stores are contiguous runs of 64
+  for (int p = threadIdx.x; p < sp.P; p += NT) {
+    int j1, j2;
+    const int j23 = qs_divmod(p, sp.n_p1, rcp1, j1);
+    const int j3 = qs_divmod(j23, sp.n_p2, rcp2, j2);
     const double v1 = centre(sp.p1s, j1, sp.dp1);
     const double v2 = centre(sp.p2s, j2, sp.dp2);
     const double v3 = centre(sp.p3s, j3, sp.dp3);
-    const bool pos1 = __builtin_amdgcn_readfirstlane(int(v1 > 0.0)) != 0;
-    const bool pos2 = __builtin_amdgcn_readfirstlane(int(v2 > 0.0)) != 0;
-    const int at = j1 + pitch1 * (j2 + sp.n_p2 * j3);
+    const int at = j1 + pitch1 * j23;
     const float f = blk[at];
     float t1 = 0.0f, t2 = 0.0f, s1 = f, s2 = f;
     if (sp.n_q1 > 1) {
       float a[7];
 #pragma unroll
-      for (int k = 3 - r; k <= 3 + r; ++k) a[k] = k == 3 ? f : u[nb1[k] + p];
-      t1 = upwind_flux_diff<RQ, float>(a, float(v1), pos1, s1);
+      for (int k = 0; k < 7; ++k) a[k] = f;
+#pragma unroll
+      for (int k = 3 - r; k <= 3 + r; ++k)
+        if (k != 3) a[k] = u[nb1[k] + p];
+      t1 = qs_q_flux_diff<RQ>(a, float(v1), float(v1) > 0.0f, s1);
     }
     if (sp.n_q2 > 1) {
       float a[7];
 #pragma unroll
-      for (int k = 3 - r; k <= 3 + r; ++k) a[k] = k == 3 ? f : u[nb2[k] + p];
-      t2 = upwind_flux_diff<RQ, float>(a, float(v2), pos2, s2);
+      for (int k = 0; k < 7; ++k) a[k] = f;
+#pragma unroll
+      for (int k = 3 - r; k <= 3 + r; ++k)
+        if (k != 3) a[k] = u[nb2[k] + p];
+      t2 = qs_q_flux_diff<RQ>(a, float(v2), float(v2) > 0.0f, s2);
     }
-    if (valid && sp.Jout) {
+    if (sp.Jout) {
       jac[0] += double(s1) * v1;
       jac[1] += double(s2) * v2;
       jac[2] += double(f) * v3;
@@ -289,15 +329,13 @@ __global__ __launch_bounds__(NT) void qsweep_kernel(QCParams qp) {
     const float wval = w_is_u ? f : wv[base + p];
     const float tv = wval + d * tcdt;
     const float x = use_un ? ta * unv[base + p] + tb * tv : (sp.b == 1.0 ? tv : tb * tv);
-    if (valid) {
-      outv[base + p] = x;
-      const double xd = double(x);
-      mac[0] += xd;
-      mac[1] += xd * v1;
-      mac[2] += xd * v2;
-      mac[3] += xd * v3;
-      mac[4] += (0.5 * xd) * ((v1 * v1 + v2 * v2) + v3 * v3);
-    }
+    outv[base + p] = x;
+    const double xd = double(x);
+    mac[0] += xd;
+    mac[1] += xd * v1;
+    mac[2] += xd * v2;
+    mac[3] += xd * v3;
+    mac[4] += (0.5 * xd) * ((v1 * v1 + v2 * v2) + v3 * v3);
   }
   qc_finish<NW>(sp, q, mac, jac, red, wave, lane);
 }
