@@ -75,6 +75,91 @@ __host__ __device__ inline double face_partial(const double* w, int s, bool righ
   return face_value<R, Dual>(y[0], y[1], y[2], y[3], y[4], right).d;
 }
 
+// WENO5 (weno5.py:58-96 as the forward kernels evaluate it, FP64): the
+// derivatives of the five upwind face states whose windows hold the gather
+// cell w[4] with respect to it, analytically and with the shared parts
+// computed once.  Face f of the gather has window w[f..f+4] and indicators
+// b_k = beta(w[f+k-1..f+k+1]) of the seven windows starting at w[0..6]; with
+// r_k = 1 / (16 beta_k^2) its weights are W = (c1 r_f, 10 r_{f+1}, c3 r_{f+2})
+// ((c1, c3) = (1, 5) for a right face, (5, 1) for a left one) and its state is
+// s = sum W_k u_k / sum W_k, so
+//   ds/dw4 = (sum_k dW_k (u_k - s) + W_k du_k) / sum W_k,
+//   dW_k = -2 W_k dbeta_k / beta_k,
+// where only the indicators of the windows starting at w[2], w[3], w[4]
+// depend on w[4].
+__host__ __device__ inline double beta_da(double a, double b, double c) {
+  return (8.0 / 3.0) * a - (19.0 / 3.0) * b + (11.0 / 3.0) * c + (a < 0.0 ? -1e-17 : 1e-17);
+}
+__host__ __device__ inline double beta_db(double a, double b, double c) {
+  return -(19.0 / 3.0) * a + (50.0 / 3.0) * b - (31.0 / 3.0) * c + (b < 0.0 ? -1e-17 : 1e-17);
+}
+__host__ __device__ inline double beta_dc(double a, double b, double c) {
+  return (11.0 / 3.0) * a - (31.0 / 3.0) * b + (20.0 / 3.0) * c + (c < 0.0 ? -1e-17 : 1e-17);
+}
+
+// Coefficient of window entry e (0..4) in substencil k (1..3) of a right or
+// left face (weno5.py:86-92).
+template <bool RIGHT>
+__host__ __device__ constexpr double weno_sub_coef(int k, int e) {
+  if (k == 1) return e == 0 ? (RIGHT ? 0.375 : -0.125) : e == 1 ? (RIGHT ? -1.25 : 0.75)
+                   : e == 2 ? (RIGHT ? 1.875 : 0.375) : 0.0;
+  if (k == 2) return e == 1 ? (RIGHT ? -0.125 : 0.375) : e == 2 ? 0.75
+                   : e == 3 ? (RIGHT ? 0.375 : -0.125) : 0.0;
+  return e == 2 ? (RIGHT ? 0.375 : 1.875) : e == 3 ? (RIGHT ? 0.75 : -1.25)
+         : e == 4 ? (RIGHT ? -0.125 : 0.375) : 0.0;
+}
+
+// One face of the gather (window w[F..F+4], gather cell at entry E = 4 - F),
+// with only the terms that do not vanish: indicator partials of the windows
+// starting at w[2..4] (G[c] = -2 dbeta_c / beta_c, so dW = W G), substencil
+// coefficients of entry E.
+template <bool RIGHT, int F>
+__host__ __device__ inline double weno_gather_face(const double* w, const double* Rk,
+                                                   const double* G) {
+  constexpr double c1 = RIGHT ? 1.0 : 5.0, c3 = RIGHT ? 5.0 : 1.0;
+  constexpr int E = 4 - F;
+  const double* y = w + F;
+  const double u1 = weno_sub_coef<RIGHT>(1, 0) * y[0] + weno_sub_coef<RIGHT>(1, 1) * y[1] +
+                    weno_sub_coef<RIGHT>(1, 2) * y[2];
+  const double u2 = weno_sub_coef<RIGHT>(2, 1) * y[1] + weno_sub_coef<RIGHT>(2, 2) * y[2] +
+                    weno_sub_coef<RIGHT>(2, 3) * y[3];
+  const double u3 = weno_sub_coef<RIGHT>(3, 2) * y[2] + weno_sub_coef<RIGHT>(3, 3) * y[3] +
+                    weno_sub_coef<RIGHT>(3, 4) * y[4];
+  const double W1 = c1 * Rk[F], W2 = 10.0 * Rk[F + 1], W3 = c3 * Rk[F + 2];
+  const double rD = fast_rcp(W1 + W2 + W3);
+  const double sv = (W1 * u1 + W2 * u2 + W3 * u3) * rD;
+  double num = 0.0;
+  if constexpr (F >= 2 && F <= 4) num += W1 * G[F] * (u1 - sv);
+  if constexpr (F + 1 >= 2 && F + 1 <= 4) num += W2 * G[F + 1] * (u2 - sv);
+  if constexpr (F + 2 >= 2 && F + 2 <= 4) num += W3 * G[F + 2] * (u3 - sv);
+  if constexpr (weno_sub_coef<RIGHT>(1, E) != 0.0) num += W1 * weno_sub_coef<RIGHT>(1, E);
+  if constexpr (weno_sub_coef<RIGHT>(2, E) != 0.0) num += W2 * weno_sub_coef<RIGHT>(2, E);
+  if constexpr (weno_sub_coef<RIGHT>(3, E) != 0.0) num += W3 * weno_sub_coef<RIGHT>(3, E);
+  return num * rD;
+}
+
+template <bool RIGHT>
+__host__ __device__ inline double weno_gather(const double* w, const double* m, double cdq) {
+  double B[7], Rk[7], G[5];
+#pragma unroll
+  for (int c = 0; c < 7; ++c) {
+    B[c] = weno_beta<double>(w[c], w[c + 1], w[c + 2]);
+    Rk[c] = fast_rcp(16.0 * B[c] * B[c]);
+  }
+  G[2] = -2.0 * beta_dc(w[2], w[3], w[4]) * fast_rcp(B[2]);
+  G[3] = -2.0 * beta_db(w[3], w[4], w[5]) * fast_rcp(B[3]);
+  G[4] = -2.0 * beta_da(w[4], w[5], w[6]) * fast_rcp(B[4]);
+  G[0] = G[1] = 0.0;  // never read
+  // face f is the lower face of cell j + f - 1 (C > 0) or j + f - 2
+  constexpr int o0 = RIGHT ? -1 : -2;
+  double acc = cdq * (m[o0 + 3] - m[o0 + 2]) * weno_gather_face<RIGHT, 0>(w, Rk, G);
+  acc += cdq * (m[o0 + 4] - m[o0 + 3]) * weno_gather_face<RIGHT, 1>(w, Rk, G);
+  acc += cdq * (m[o0 + 5] - m[o0 + 4]) * weno_gather_face<RIGHT, 2>(w, Rk, G);
+  acc += cdq * (m[o0 + 6] - m[o0 + 5]) * weno_gather_face<RIGHT, 3>(w, Rk, G);
+  acc += cdq * (m[o0 + 7] - m[o0 + 6]) * weno_gather_face<RIGHT, 4>(w, Rk, G);
+  return acc;
+}
+
 // Gather of the advection adjoint along one axis for the cell at window
 // centre: w[0..8] = u at offsets -4..4, m[0..6] = mu at offsets -3..3,
 // cdq = C/dq.  Upwind (C > 0): the lower face of cell k is the right face of
@@ -82,6 +167,11 @@ __host__ __device__ inline double face_partial(const double* w, int s, bool righ
 // cell k, stencil k-2..k+2, for k = j-2..j+2.
 template <int R>
 __host__ __device__ inline double axis_adjoint(const double* w, const double* m, double cdq) {
+  if constexpr (R == RC_WENO5) {
+    if (cdq > 0.0) return weno_gather<true>(w, m, cdq);
+    if (cdq < 0.0) return weno_gather<false>(w, m, cdq);
+    return 0.0;
+  }
   double acc = 0.0;
   if (cdq > 0.0) {
 #pragma unroll
@@ -282,24 +372,33 @@ __global__ __launch_bounds__(256) void adj_bgk_reduce_kernel(AdjParams ap) {
 template <int R>
 __host__ __device__ inline void adj_stage_at(const AdjParams& ap, int i1, int i2, int p, long t) {
   const long Qn = (long)ap.n1 * ap.n2;
-  const int j1 = p % ap.n_p1, j2 = (p / ap.n_p1) % ap.n_p2, j3 = p / (ap.n_p1 * ap.n_p2);
+  int j1, j2;
+  const int j23 = fdivmod(p, ap.n_p1, 1.0f / float(ap.n_p1), j1);
+  const int j3 = fdivmod(j23, ap.n_p2, 1.0f / float(ap.n_p2), j2);
   const double v1 = centre(ap.p1s, j1, ap.dp1), v2 = centre(ap.p2s, j2, ap.dp2);
   const double v3 = centre(ap.p3s, j3, ap.dp3);
   const double mu = ap.lam[t];
   double acc = 0.0;
   double w[9], m[7];
+  // neighbours along q1 are s1 elements apart in memory, along q2 n1*P (both
+  // layouts keep i2 outermost); offsets reach +-4 cells, one wrap for n >= 5
+  const long s1 = ap.layout ? 1 : ap.P, s2 = (long)ap.n1 * ap.P;
   if (ap.n1 > 1) {
 #pragma unroll
-    for (int r = 0; r < 9; ++r) w[r] = ap.u[adj_index(ap, qm_mod(i1 + r - 4, ap.n1), i2, p)];
-#pragma unroll
-    for (int r = 0; r < 7; ++r) m[r] = ap.lam[adj_index(ap, qm_mod(i1 + r - 3, ap.n1), i2, p)];
+    for (int r = 0; r < 9; ++r) {
+      const int i = ap.n1 >= 5 ? wrap_once(i1 + r - 4, ap.n1) : qm_mod(i1 + r - 4, ap.n1);
+      w[r] = ap.u[t + (long)(i - i1) * s1];
+      if (r >= 1 && r <= 7) m[r - 1] = ap.lam[t + (long)(i - i1) * s1];
+    }
     acc += axis_adjoint<R>(w, m, v1 * ap.inv_dq1);
   }
   if (ap.n2 > 1) {
 #pragma unroll
-    for (int r = 0; r < 9; ++r) w[r] = ap.u[adj_index(ap, i1, qm_mod(i2 + r - 4, ap.n2), p)];
-#pragma unroll
-    for (int r = 0; r < 7; ++r) m[r] = ap.lam[adj_index(ap, i1, qm_mod(i2 + r - 3, ap.n2), p)];
+    for (int r = 0; r < 9; ++r) {
+      const int i = ap.n2 >= 5 ? wrap_once(i2 + r - 4, ap.n2) : qm_mod(i2 + r - 4, ap.n2);
+      w[r] = ap.u[t + (long)(i - i2) * s2];
+      if (r >= 1 && r <= 7) m[r - 1] = ap.lam[t + (long)(i - i2) * s2];
+    }
     acc += axis_adjoint<R>(w, m, v2 * ap.inv_dq2);
   }
   if (ap.source) {

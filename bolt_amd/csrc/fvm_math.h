@@ -320,6 +320,28 @@ __host__ __device__ inline int qm_mod(int i, int n) {
   return i < 0 ? i + n : i;
 }
 
+// The same for -n <= i < 2n (one conditional wrap).
+__host__ __device__ inline int wrap_once(int i, int n) {
+  return i < 0 ? i + n : (i >= n ? i - n : i);
+}
+
+// x / d (returned) and x % d (in rem) for 0 <= x < 2^24, 0 < d < 2^24, through
+// the FP32 reciprocal rcp_d = 1/d: the quotient estimate is off by at most
+// one and is corrected, instead of an integer division sequence.
+__host__ __device__ inline int fdivmod(int x, int d, float rcp_d, int& rem) {
+  int qt = int(float(x) * rcp_d);
+  int r = x - qt * d;
+  if (r < 0) {
+    --qt;
+    r += d;
+  } else if (r >= d) {
+    ++qt;
+    r -= d;
+  }
+  rem = r;
+  return qt;
+}
+
 // Cell-centre coordinate, calculate_p.py:48-50: start + (0.5 + i) * d.
 __host__ __device__ inline double centre(double start, int i, double d) {
   return start + (0.5 + double(i)) * d;

@@ -36,21 +36,9 @@ namespace bx {
 
 constexpr int QS_SEG = 8;  // cells per sweep segment
 
-// x / d and x % d for 0 <= x < 2^24 and 0 < d < 2^24 through the FP32
-// reciprocal rcp_d = 1/d (the quotient estimate is off by at most one and
-// corrected), instead of an integer division sequence.
+// x / d and x % d through the FP32 reciprocal (fvm_math.h).
 __device__ __forceinline__ int qs_divmod(int x, int d, float rcp_d, int& rem) {
-  int qt = int(float(x) * rcp_d);
-  int r = x - qt * d;
-  if (r < 0) {
-    --qt;
-    r += d;
-  } else if (r >= d) {
-    ++qt;
-    r -= d;
-  }
-  rem = r;
-  return qt;
+  return fdivmod(x, d, rcp_d, rem);
 }
 
 // The reference's WENO5 indicator (sum-of-squares form of weno5.py:58-71) as
