@@ -193,10 +193,11 @@ def copy_bandwidth(mib=1024, reps=10):
     return 2 * n * 8 / s / 1e9
 
 
-def run_adjoint(steps=100, every=10):
+def run_adjoint(steps=100, every=10, repeats=3):
     """The adjoint workload: 2D2V 64^2 x 32^2, WENO5 + LLF + BGK tau = 0.05,
     SSP-RK3 dt = 5e-4, 100 steps with checkpoints every 10.  Times the
-    forward run and the backward run (segment recompute + stage adjoints)."""
+    forward run and the backward run (segment recompute + stage adjoints)
+    after one untimed run; the medians of ``repeats`` runs are reported."""
     import torch
     from bolt_amd import adjoint as A
     from tests.cases import case_c4, hip_solver, maxwellian_interior
@@ -218,14 +219,21 @@ def run_adjoint(steps=100, every=10):
         return t1, time.perf_counter()
 
     once()  # warm-up
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    t1, t2 = once()
+    fwd, bwd = [], []
+    for _ in range(repeats):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        t1, t2 = once()
+        fwd.append((t1 - t0) * 1e3)
+        bwd.append((t2 - t1) * 1e3)
     cells = s._Q * s._P
+    f_ms, b_ms = _median(fwd), _median(bwd)
     return {"workload": "2D2V adjoint 64^2 q x 32^2 p, WENO5 + LLF + BGK tau=0.05, SSP-RK3 "
                         "dt=5e-4, %d steps, checkpoints every %d, FP64" % (steps, every),
-            "forward_ms": (t1 - t0) * 1e3, "backward_ms": (t2 - t1) * 1e3,
-            "cell_steps_per_s_forward_plus_backward": cells * steps / (t2 - t0)}
+            "forward_ms": f_ms, "backward_ms": b_ms,
+            "forward_ms_repeats": fwd, "backward_ms_repeats": bwd,
+            "backward_over_forward": b_ms / f_ms,
+            "cell_steps_per_s_forward_plus_backward": cells * steps / ((f_ms + b_ms) / 1e3)}
 
 
 def run_em(steps=10, warmup=2):
