@@ -359,8 +359,7 @@ def run_ensemble(steps=200, sample_steps=10):
             "roofline": {"bound": "hbm", "level": "step (all launches of a step)",
                          "achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": step_gbs / HBM_PEAK_GBS, "traffic": None},
-            "tuning": {"waves_per_group": ens.tuning[0], "cells_per_task": ens.tuning[1],
-                       "groups_per_cu": ens.tuning[2], "chunk_cells": ens.tuning[3]},
+            "tuning": {"chunk_cells": ens.tuning[0], "batch_cells": ens.tuning[1]},
             "autotune_s": t_tune,
             "sequential_fp64": {"value": seq, "sample": "%d steps per member, one member "
                                                         "after another" % sample_steps},

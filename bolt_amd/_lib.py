@@ -80,8 +80,7 @@ class EnsMember(ctypes.Structure):
 
 
 class EnsTuning(ctypes.Structure):
-    _fields_ = [("waves_per_group", ctypes.c_int32), ("cells_per_task", ctypes.c_int32),
-                ("groups_per_cu", ctypes.c_int32), ("chunk_cells", ctypes.c_int32)]
+    _fields_ = [("chunk_cells", ctypes.c_int32), ("batch_cells", ctypes.c_int32)]
 
 
 class BoltxError(RuntimeError):

@@ -1,9 +1,12 @@
-// C-ABI of the ensemble step (declarations in include/boltx.h): work-list
-// planning on the host, the persistent stage launch, and the batched moment
-// and Maxwellian-table launches.
+// C-ABI of the ensemble step (declarations in include/boltx.h): chunk-list
+// planning on the host, the chunk stage launch (stage and partial moments of
+// its output), the batched moment-block / Maxwellian-table launch, and the
+// moment launch for a given state.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -17,14 +20,11 @@ namespace {
 
 struct Plan {
   std::vector<EnsInst> inst;
-  std::vector<EnsTile> tiles;
-  std::vector<int> wg_begin;
-  std::vector<EnsChunk> chunks;
+  std::vector<EnsChunk> chunks;  // in launch order: largest first
   long pooled = 0, tab = 0, mom = 0, part = 0;
-  int groups = 0;
+  size_t lds = 0;                // dynamic LDS bytes of the chunk kernel
   // byte offsets inside the workspace
-  size_t o_inst = 0, o_tiles = 0, o_wg = 0, o_chunks = 0, o_mom = 0, o_part = 0, o_tab = 0,
-         o_flag = 0, total = 0;
+  size_t o_inst = 0, o_chunks = 0, o_mom = 0, o_part = 0, o_tab = 0, o_flag = 0, total = 0;
 };
 
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
@@ -53,31 +53,16 @@ int check_members(int n, const boltx_ens_member* m) {
 
 int check_tuning(const boltx_ens_tuning* t) {
   if (!t) return fail(BOLTX_EINVAL, "ensemble: tuning is NULL");
-  if (t->waves_per_group != 1 && t->waves_per_group != 2 && t->waves_per_group != 4)
-    return fail(BOLTX_EINVAL, "waves_per_group must be 1, 2 or 4");
-  if (t->cells_per_task < 1 || t->groups_per_cu < 1 || t->groups_per_cu > 64 || t->chunk_cells < 1)
-    return fail(BOLTX_EINVAL, "ensemble tuning out of range");
+  if (t->chunk_cells < 1 || t->batch_cells < 1)
+    return fail(BOLTX_EINVAL, "ensemble tuning: chunk_cells and batch_cells must be positive");
   return BOLTX_OK;
 }
 
-int cu_count() {
-  int dev = 0, n = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return 256;
-  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
-    return 256;
-  return n;
-}
-
-// Work lists and workspace map of an ensemble under a tuning.
-Plan make_plan(int n, const boltx_ens_member* m, const boltx_ens_tuning* t, int ncu) {
+// Chunk list and workspace map of an ensemble under a tuning.
+Plan make_plan(int n, const boltx_ens_member* m, const boltx_ens_tuning* t) {
   Plan pl;
   const int esz = m[0].phys.precision == BOLTX_FP64 ? 8 : 4;
-  // members in order of decreasing line length keep one line shape per
-  // stretch of the work list
-  std::vector<int> order(n);
-  for (int i = 0; i < n; ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int a, int b) { return m[a].grid.n_q1 > m[b].grid.n_q1; });
+  const int vw = 16 / esz;  // cells per 16-byte vector
   pl.inst.resize(n);
   for (int i = 0; i < n; ++i) {
     const boltx_grid& g = m[i].grid;
@@ -88,22 +73,32 @@ Plan make_plan(int n, const boltx_ens_member* m, const boltx_ens_tuning* t, int 
     I.n_p1 = g.n_p1;
     I.n_p2 = g.n_p2;
     I.n_p3 = g.n_p3;
+    I.rn_p1 = 1.0f / float(g.n_p1);
+    I.rn_p2 = 1.0f / float(g.n_p2);
     I.P = g.n_p1 * g.n_p2 * g.n_p3;
     I.p_dim = ph.p_dim;
     I.source = ph.source_enabled;
-    I.lanes = I.n1 <= 64 ? 64 / I.n1 : 1;
     I.off_f = pl.pooled;
     pl.pooled += (long)I.P * I.n1;
-    I.off_tab = pl.tab;
-    pl.tab += (long)(g.n_p1 + g.n_p2 + g.n_p3) * I.n1;
+    // table blocks start on 16-byte boundaries
+    I.off_tab = (pl.tab + vw - 1) / vw * vw;
+    pl.tab = I.off_tab + (long)(g.n_p1 + g.n_p2 + g.n_p3) * I.n1;
+    // items of two 16-byte vectors where the rows allow them
+    I.w = I.n1 % (2 * vw) == 0 && I.off_f % vw == 0 ? 2 * vw : 1;
     I.off_mom = pl.mom;
     pl.mom += (long)MOM_ROWS * I.n1;
-    const int pc = std::max(1, t->chunk_cells / I.n1);
-    I.nchunk = (I.P + pc - 1) / pc;
+    // whole q1 lines per chunk and per batch; a batch is at most
+    // ENS_THREADS items of I.w cells
+    const int lc = std::min(I.P, std::max(1, t->chunk_cells / I.n1));
+    I.batch = std::max(1, std::min({t->batch_cells / I.n1, ENS_THREADS / (I.n1 / I.w), lc}));
+    I.nchunk = (I.P + lc - 1) / lc;
     I.off_part = pl.part;
     pl.part += (long)I.nchunk * 5 * I.n1;
     for (int c = 0; c < I.nchunk; ++c)
-      pl.chunks.push_back({i, c, c * pc, std::min(I.P, (c + 1) * pc)});
+      pl.chunks.push_back({i, c, c * lc, std::min(I.P, (c + 1) * lc)});
+    const size_t lds = esz == 8 ? ens_chunk_lds<double>(I.n1, I.batch)
+                                : ens_chunk_lds<float>(I.n1, I.batch);
+    pl.lds = std::max(pl.lds, lds);
     I.inv_dq = 1.0 / g.dq1;
     I.cdt = m[i].dt;
     I.inv_tau = ph.source_enabled ? 1.0 / ph.tau : 0.0;
@@ -117,43 +112,14 @@ Plan make_plan(int n, const boltx_ens_member* m, const boltx_ens_tuning* t, int 
     I.mass = ph.mass;
     I.kB = ph.boltzmann_constant;
   }
-  // wave tasks of about cells_per_task cells (whole waves of packed lines)
-  std::vector<long> cost;
-  for (int oi = 0; oi < n; ++oi) {
-    const int i = order[oi];
-    const EnsInst& I = pl.inst[i];
-    int np = std::max(1, t->cells_per_task / I.n1);
-    if (I.n1 <= 64) np = (np + I.lanes - 1) / I.lanes * I.lanes;
-    for (int p0 = 0; p0 < I.P; p0 += np) {
-      const int k = std::min(np, I.P - p0);
-      pl.tiles.push_back({i, p0, k, 0});
-      // a wave pays for whole 64-lane rows
-      const long rows = I.n1 <= 64 ? (k + I.lanes - 1) / I.lanes : (long)k * ((I.n1 + 63) / 64);
-      cost.push_back(rows);
-    }
-  }
-  // contiguous ranges of equal cost for the persistent workgroups
-  long total = 0;
-  for (long c : cost) total += c;
-  const int G = std::max(1, std::min<int>((int)pl.tiles.size(), t->groups_per_cu * ncu));
-  pl.groups = G;
-  pl.wg_begin.assign(G + 1, (int)pl.tiles.size());
-  pl.wg_begin[0] = 0;
-  long acc = 0;
-  int g = 1;
-  for (size_t k = 0; k < pl.tiles.size() && g < G; ++k) {
-    acc += cost[k];
-    while (g < G && acc * G >= total * g) pl.wg_begin[g++] = (int)k + 1;
-  }
-  for (; g < G; ++g) pl.wg_begin[g] = (int)pl.tiles.size();
+  // launch the largest chunks first, so the small ones fill the tail
+  std::stable_sort(pl.chunks.begin(), pl.chunks.end(), [&](const EnsChunk& a, const EnsChunk& b) {
+    return (long)(a.p1 - a.p0) * pl.inst[a.inst].n1 > (long)(b.p1 - b.p0) * pl.inst[b.inst].n1;
+  });
   // workspace map
   size_t o = 0;
   pl.o_inst = o;
   o = align256(o + pl.inst.size() * sizeof(EnsInst));
-  pl.o_tiles = o;
-  o = align256(o + pl.tiles.size() * sizeof(EnsTile));
-  pl.o_wg = o;
-  o = align256(o + pl.wg_begin.size() * sizeof(int));
   pl.o_chunks = o;
   o = align256(o + pl.chunks.size() * sizeof(EnsChunk));
   pl.o_mom = o;
@@ -168,64 +134,90 @@ Plan make_plan(int n, const boltx_ens_member* m, const boltx_ens_tuning* t, int 
   return pl;
 }
 
+// The chunk kernel reads and writes 16-byte vectors of the pooled arrays and
+// of the workspace tables.
+int check_aligned(const void* p, const char* what) {
+  if ((uintptr_t)p % 16)
+    return fail(BOLTX_EINVAL, "ensemble: %s must be 16-byte aligned", what);
+  return BOLTX_OK;
+}
+
 template <typename T>
-int moments_and_tables(const Plan& pl, char* ws, const void* f, hipStream_t s) {
-  const EnsInst* inst = (const EnsInst*)(ws + pl.o_inst);
-  hipLaunchKernelGGL(ens_moments_kernel<T>, dim3((unsigned)pl.chunks.size()), dim3(256), 0, s,
-                     inst, (const EnsChunk*)(ws + pl.o_chunks), (const T*)f,
-                     (double*)(ws + pl.o_part));
-  int rc = launch_status("ens_moments_kernel");
-  if (rc) return rc;
+int finalize(const Plan& pl, char* ws, hipStream_t s) {
   hipLaunchKernelGGL(ens_finalize_kernel<T>, dim3((unsigned)pl.inst.size(), ENS_TGROUPS), dim3(256), 0, s,
-                     inst, (const double*)(ws + pl.o_part), (double*)(ws + pl.o_mom),
-                     (T*)(ws + pl.o_tab), (int*)(ws + pl.o_flag));
+                     (const EnsInst*)(ws + pl.o_inst), (const double*)(ws + pl.o_part),
+                     (double*)(ws + pl.o_mom), (T*)(ws + pl.o_tab), (int*)(ws + pl.o_flag));
   return launch_status("ens_finalize_kernel");
 }
 
 template <typename T>
-int stage(const Plan& pl, char* ws, int recon, int wpg, const void* u, const void* w,
-          const void* un, void* out, double a, double b, double cscale, hipStream_t s) {
+int moments_and_tables(const Plan& pl, char* ws, const void* f, hipStream_t s) {
+  hipLaunchKernelGGL(ens_moments_kernel<T>, dim3((unsigned)pl.chunks.size()), dim3(256), 0, s,
+                     (const EnsInst*)(ws + pl.o_inst), (const EnsChunk*)(ws + pl.o_chunks),
+                     (const T*)f, (double*)(ws + pl.o_part));
+  int rc = launch_status("ens_moments_kernel");
+  if (rc) return rc;
+  return finalize<T>(pl, ws, s);
+}
+
+template <typename T>
+int stage(const Plan& pl, char* ws, int recon, const void* u, const void* w, const void* un,
+          void* out, double a, double b, double cscale, hipStream_t s) {
   EnsStageArgs ea;
   ea.inst = (const EnsInst*)(ws + pl.o_inst);
-  ea.tiles = (const EnsTile*)(ws + pl.o_tiles);
-  ea.wg_begin = (const int*)(ws + pl.o_wg);
+  ea.chunks = (const EnsChunk*)(ws + pl.o_chunks);
   ea.u = u;
   ea.w = w;
   ea.un = un;
   ea.out = out;
   ea.tab = ws + pl.o_tab;
+  ea.part = (double*)(ws + pl.o_part);
   ea.a = a;
   ea.b = b;
   ea.cscale = cscale;
   ea.use_un = a != 0.0;
   ea.b_one = b == 1.0;
-  const dim3 grid(pl.groups), block(64 * wpg);
+  const dim3 grid((unsigned)pl.chunks.size()), block(ENS_THREADS);
+  const unsigned lds = (unsigned)pl.lds;
+  const int form = w == u && a == 0.0 && b == 1.0 ? ENS_FORM_FIRST
+                   : w == u && a != 0.0           ? ENS_FORM_COMBINE
+                                                  : ENS_FORM_GENERAL;
+  auto go = [&](auto rc_, auto form_) {
+    hipLaunchKernelGGL((ens_chunk_kernel<T, decltype(rc_)::value, decltype(form_)::value>), grid,
+                       block, lds, s, ea);
+  };
+  auto by_form = [&](auto rc_) {
+    using F0 = std::integral_constant<int, ENS_FORM_FIRST>;
+    using F1 = std::integral_constant<int, ENS_FORM_COMBINE>;
+    using F2 = std::integral_constant<int, ENS_FORM_GENERAL>;
+    if (form == ENS_FORM_FIRST) go(rc_, F0());
+    else if (form == ENS_FORM_COMBINE) go(rc_, F1());
+    else go(rc_, F2());
+  };
   switch (recon) {
-    case RC_PC: hipLaunchKernelGGL((ens_stage_kernel<T, RC_PC>), grid, block, 0, s, ea); break;
-    case RC_MINMOD:
-      hipLaunchKernelGGL((ens_stage_kernel<T, RC_MINMOD>), grid, block, 0, s, ea);
-      break;
-    case RC_PPM: hipLaunchKernelGGL((ens_stage_kernel<T, RC_PPM>), grid, block, 0, s, ea); break;
-    default: hipLaunchKernelGGL((ens_stage_kernel<T, RC_WENO5>), grid, block, 0, s, ea);
+    case RC_PC: by_form(std::integral_constant<int, RC_PC>()); break;
+    case RC_MINMOD: by_form(std::integral_constant<int, RC_MINMOD>()); break;
+    case RC_PPM: by_form(std::integral_constant<int, RC_PPM>()); break;
+    default: by_form(std::integral_constant<int, RC_WENO5>());
   }
-  int rc = launch_status("ens_stage_kernel");
+  int rc = launch_status("ens_chunk_kernel");
   if (rc) return rc;
-  return moments_and_tables<T>(pl, ws, out, s);
+  return finalize<T>(pl, ws, s);
 }
 
 template <typename T>
-int steps(const Plan& pl, char* ws, int recon, int wpg, int stepper, void* f, void* u1, void* u2,
+int steps(const Plan& pl, char* ws, int recon, int stepper, void* f, void* u1, void* u2,
           int n_steps, hipStream_t s) {
   int rc = BOLTX_OK;
   for (int k = 0; k < n_steps && !rc; ++k) {
     if (stepper == BOLTX_RK2) {
       // fvm_operator.py:25-33: u1 = f + dt/2 L(f); f = f + dt L(u1)
-      if ((rc = stage<T>(pl, ws, recon, wpg, f, f, nullptr, u1, 0.0, 1.0, 0.5, s))) break;
-      rc = stage<T>(pl, ws, recon, wpg, u1, f, nullptr, f, 0.0, 1.0, 1.0, s);
+      if ((rc = stage<T>(pl, ws, recon, f, f, nullptr, u1, 0.0, 1.0, 0.5, s))) break;
+      rc = stage<T>(pl, ws, recon, u1, f, nullptr, f, 0.0, 1.0, 1.0, s);
     } else {
-      if ((rc = stage<T>(pl, ws, recon, wpg, f, f, nullptr, u1, 0.0, 1.0, 1.0, s))) break;
-      if ((rc = stage<T>(pl, ws, recon, wpg, u1, u1, f, u2, 0.75, 0.25, 1.0, s))) break;
-      rc = stage<T>(pl, ws, recon, wpg, u2, u2, f, f, 1.0 / 3.0, 2.0 / 3.0, 1.0, s);
+      if ((rc = stage<T>(pl, ws, recon, f, f, nullptr, u1, 0.0, 1.0, 1.0, s))) break;
+      if ((rc = stage<T>(pl, ws, recon, u1, u1, f, u2, 0.75, 0.25, 1.0, s))) break;
+      rc = stage<T>(pl, ws, recon, u2, u2, f, f, 1.0 / 3.0, 2.0 / 3.0, 1.0, s);
     }
   }
   return rc;
@@ -251,7 +243,7 @@ extern "C" long long boltx_ensemble_layout(int n, const boltx_ens_member* member
 extern "C" size_t boltx_ensemble_workspace_size(int n, const boltx_ens_member* members,
                                                 const boltx_ens_tuning* tuning) {
   if (check_members(n, members) || check_tuning(tuning)) return 0;
-  return make_plan(n, members, tuning, cu_count()).total;
+  return make_plan(n, members, tuning).total;
 }
 
 extern "C" int boltx_ensemble_map(int n, const boltx_ens_member* members,
@@ -259,7 +251,7 @@ extern "C" int boltx_ensemble_map(int n, const boltx_ens_member* members,
                                   size_t* flag_offset) {
   int rc = check_members(n, members);
   if (rc || (rc = check_tuning(tuning))) return rc;
-  const Plan pl = make_plan(n, members, tuning, cu_count());
+  const Plan pl = make_plan(n, members, tuning);
   if (moments_offset) *moments_offset = pl.o_mom;
   if (flag_offset) *flag_offset = pl.o_flag;
   return BOLTX_OK;
@@ -271,7 +263,8 @@ extern "C" int boltx_ensemble_init(int n, const boltx_ens_member* members,
   int rc = check_members(n, members);
   if (rc || (rc = check_tuning(tuning))) return rc;
   if (!f || !workspace) return fail(BOLTX_EINVAL, "ensemble: f or workspace is NULL");
-  const Plan pl = make_plan(n, members, tuning, cu_count());
+  if ((rc = check_aligned(f, "f")) || (rc = check_aligned(workspace, "the workspace"))) return rc;
+  const Plan pl = make_plan(n, members, tuning);
   char* ws = (char*)workspace;
   hipStream_t s = (hipStream_t)stream;
   struct Piece {
@@ -279,8 +272,6 @@ extern "C" int boltx_ensemble_init(int n, const boltx_ens_member* members,
     const void* src;
     size_t bytes;
   } pieces[] = {{pl.o_inst, pl.inst.data(), pl.inst.size() * sizeof(EnsInst)},
-                {pl.o_tiles, pl.tiles.data(), pl.tiles.size() * sizeof(EnsTile)},
-                {pl.o_wg, pl.wg_begin.data(), pl.wg_begin.size() * sizeof(int)},
                 {pl.o_chunks, pl.chunks.data(), pl.chunks.size() * sizeof(EnsChunk)}};
   for (const Piece& pc : pieces)
     if (hipMemcpyAsync(ws + pc.off, pc.src, pc.bytes, hipMemcpyHostToDevice, s) != hipSuccess)
@@ -303,11 +294,14 @@ extern "C" int boltx_ensemble_step(int n, const boltx_ens_member* members,
     return fail(BOLTX_EINVAL, "ensemble: state or workspace pointers missing");
   if (stepper != BOLTX_RK2 && stepper != BOLTX_SSP_RK3)
     return fail(BOLTX_EINVAL, "ensemble: stepper %d", stepper);
-  const Plan pl = make_plan(n, members, tuning, cu_count());
+  if ((rc = check_aligned(f, "f")) || (rc = check_aligned(u1, "u1")) ||
+      (u2 && (rc = check_aligned(u2, "u2"))) || (rc = check_aligned(workspace, "the workspace")))
+    return rc;
+  const Plan pl = make_plan(n, members, tuning);
   char* ws = (char*)workspace;
   hipStream_t s = (hipStream_t)stream;
-  const int recon = members[0].phys.recon_q, wpg = tuning->waves_per_group;
+  const int recon = members[0].phys.recon_q;
   if (members[0].phys.precision == BOLTX_FP64)
-    return steps<double>(pl, ws, recon, wpg, stepper, f, u1, u2, n_steps, s);
-  return steps<float>(pl, ws, recon, wpg, stepper, f, u1, u2, n_steps, s);
+    return steps<double>(pl, ws, recon, stepper, f, u1, u2, n_steps, s);
+  return steps<float>(pl, ws, recon, stepper, f, u1, u2, n_steps, s);
 }

@@ -3,22 +3,22 @@
 // time and time step, advanced together.  Every instance keeps f in the
 // q1-line layout [P][n1] inside pooled arrays.
 //
-// One SSP-RK3 / RK2 stage of the whole ensemble is three launches:
-//  * ens_stage_kernel: a persistent grid of workgroups, each walking its own
-//    range of wave tasks (instance, p-range); a wave holds one or more
-//    periodic q1 lines (n1 <= 64: 64/n1 lines per wave; larger n1: n1/64
-//    cells per lane), exchanges the line through LDS and evaluates the
-//    reconstruction, upwind flux, BGK source and RK combination of every
-//    cell (df_dt_fvm.py:56-97, collision_operator.py:70-128,
-//    fvm_operator.py:22-33);
-//  * ens_moments_kernel: one workgroup per (instance, p-chunk), FP64
-//    partial moments in a fixed order;
-//  * ens_finalize_kernel: one workgroup per instance: the moment block
-//    (moments.py:5-22, collision_operator.py:102-116) and the separable
-//    Maxwellian tables the next stage reads.
-// The work lists are built on the host (tile sizes from the tuning
-// parameters, workgroup ranges balanced by cell count), so no kernel waits
-// on another workgroup.
+// One SSP-RK3 / RK2 stage of the whole ensemble is two launches:
+//  * ens_chunk_kernel: one workgroup per (instance, p-chunk).  A chunk is a
+//    contiguous span of whole q1 lines of one instance; the workgroup streams
+//    it through LDS in batches of lines (16-byte loads and stores wherever
+//    the rows allow, one batch's loads in flight while the previous batch's
+//    moments are summed), evaluates the reconstruction, upwind flux, BGK
+//    source and RK combination of every cell (df_dt_fvm.py:56-97,
+//    collision_operator.py:70-128, fvm_operator.py:22-33) and sums the FP64
+//    partial moments of the stage output over the chunk's lines in a fixed
+//    order (compute_moments.py:7-59), so f is not read a second time;
+//  * ens_finalize_kernel: one workgroup per instance and table group: the
+//    moment block (moments.py:5-22, collision_operator.py:102-116) from the
+//    partials, and the separable Maxwellian tables the next stage reads.
+// ens_moments_kernel computes the same partials from a given state (the
+// initial one).  The chunk list is built on the host; no kernel waits on
+// another workgroup.
 #pragma once
 #include <type_traits>
 #include "qline.h"
@@ -26,246 +26,285 @@
 namespace bx {
 
 struct EnsInst {
-  int n1, P, n_p1, n_p2, n_p3, p_dim, source, lanes;  // lanes: q1 cells per lane-group
+  int n1, P, n_p1, n_p2, n_p3, p_dim, source;
+  int w;           // q1 cells per chunk-kernel item: 2 * (16 / sizeof(T)) on vector rows, else 1
+  int batch;       // q1 lines a chunk workgroup holds in LDS at once
+  int nchunk;      // p-chunks (partial moment blocks)
+  float rn_p1, rn_p2;  // 1 / n_p1 and 1 / n_p2 for fdivmod
   long off_f;     // element offset of the [P][n1] block in the pooled f-sized arrays
   long off_tab;   // element offset of the [n_p1+n_p2+n_p3][n1] Maxwellian tables
   long off_mom;   // double offset of the [12][n1] moment block
   long off_part;  // double offset of the [nchunk][5][n1] partial moments
-  int nchunk, pad;
   double inv_dq, cdt, inv_tau, p1s, dp1, p2s, dp2, p3s, dp3, measure, mass, kB;
 };
 
-struct EnsTile {
-  int inst, p0, np, pad;  // wave task: p-cells p0 .. p0+np-1 of instance inst
-};
-
 struct EnsChunk {
-  int inst, chunk, p0, p1;  // moment workgroup: p-cells p0 .. p1-1
+  int inst, chunk, p0, p1;  // p-cells p0 .. p1-1 of instance inst; partial block ``chunk``
 };
 
 struct EnsStageArgs {
   const EnsInst* inst;
-  const EnsTile* tiles;
-  const int* wg_begin;  // tiles of workgroup g: [wg_begin[g], wg_begin[g+1])
+  const EnsChunk* chunks;
   const void* u;
   const void* w;
   const void* un;
   void* out;
   const void* tab;
+  double* part;
   double a, b, cscale;  // stage: out = a*un + b*(w + cscale*dt_i*L(u))
   int use_un, b_one;
 };
 
-// LDS per wave: q1 lines with QL_HALO periodic copies on each side; a wave
-// packs 64 / n1 lines when n1 <= 64 (at most 12 lines of n1 + 6 cells for
-// n1 >= 5) or holds one line of up to 256 cells.
-constexpr int ENS_LDS = 264;
+// One item (a run of W q1 cells of one line) per thread and batch: a batch
+// holds at most ENS_THREADS items.
+constexpr int ENS_THREADS = 256;
+// Periodic halo cells on each side of an LDS line: 4 keeps every line's
+// cells 16-byte aligned and covers the 3-cell reach of the stencil.
+constexpr int ENS_HALO = 4;
 
-// Everything the stage reads from memory for one line of a wave (the lane's
-// cells of it): u, the Maxwellian table factors, w and un, and the line's C1.
-// A wave fetches the next line into one of two such slots while it works on
-// the other, and the slots swap roles from line to line, so no loaded value
-// is copied (a copy would wait for the loads still in flight).
-template <typename T, int NR>
-struct EnsLineIn {
-  T f[NR], t1[NR], t2[NR], t3[NR], wc[NR], unc[NR];
-  T C1;
-};
+// LDS of the chunk kernel: the batch's lines of u (halo-padded), its
+// outputs [batch][n1], and the moment weights (v1, v2, v3, |v|^2 / 2) of its
+// lines in two slots (the previous batch's are read while the next batch's
+// are written).  The group partials at the end of a chunk reuse the first
+// region.
+template <typename T>
+__host__ __device__ inline size_t ens_chunk_lds_lines(int n1, int batch) {
+  size_t fo = (size_t)batch * (2 * n1 + 2 * ENS_HALO) * sizeof(T);
+  const size_t red = 5 * ENS_THREADS * sizeof(double);
+  return ((fo > red ? fo : red) + 15) / 16 * 16;
+}
+template <typename T>
+__host__ __device__ inline size_t ens_chunk_lds(int n1, int batch) {
+  return ens_chunk_lds_lines<T>(n1, batch) + 2 * (size_t)batch * 4 * sizeof(double);
+}
 
-// Fetch line p (velocity indices j1, j2, j3) for lane cells
-// lane_in_line + 64 k; nothing is loaded for dead lanes or a dead line.
-template <typename T, int NR>
-__device__ inline void ens_fetch(const EnsStageArgs& ea, const EnsInst& I, int p, int j1, int j2,
-                                 int j3, int lane_in_line, bool live_line, EnsLineIn<T, NR>& in) {
-  const int n1 = I.n1;
-  const long base = I.off_f + (long)p * n1;
+// Global and LDS accesses of N consecutive cells: 16-byte vectors when N is
+// a multiple of the vector width (the caller guarantees the alignment),
+// else one element.
+template <typename T, int N, bool STREAM>
+__device__ __forceinline__ void ens_ld(const T* p, T* x) {
+  constexpr int V = 16 / sizeof(T);
+  if constexpr (N % V != 0) {
+    static_assert(N == 1, "one element or whole vectors");
+    x[0] = STREAM ? __builtin_nontemporal_load(p) : *p;
+  } else {
+    typedef T Vt __attribute__((ext_vector_type(V)));
+#pragma unroll
+    for (int k = 0; k < N; k += V) {
+      const Vt v = STREAM ? __builtin_nontemporal_load((const Vt*)(p + k)) : *(const Vt*)(p + k);
+#pragma unroll
+      for (int r = 0; r < V; ++r) x[k + r] = v[r];
+    }
+  }
+}
+template <typename T, int N, bool STREAM>
+__device__ __forceinline__ void ens_st(T* p, const T* x) {
+  constexpr int V = 16 / sizeof(T);
+  if constexpr (N % V != 0) {
+    static_assert(N == 1, "one element or whole vectors");
+    if (STREAM) __builtin_nontemporal_store(x[0], p);
+    else *p = x[0];
+  } else {
+    typedef T Vt __attribute__((ext_vector_type(V)));
+#pragma unroll
+    for (int k = 0; k < N; k += V) {
+      Vt v;
+#pragma unroll
+      for (int r = 0; r < V; ++r) v[r] = x[k + r];
+      if (STREAM) __builtin_nontemporal_store(v, (Vt*)(p + k));
+      else *(Vt*)(p + k) = v;
+    }
+  }
+}
+
+// Velocity indices (j1, j2, j3) of p-cell p.
+__device__ __forceinline__ void ens_jjj(const EnsInst& I, int p, int& j1, int& j2, int& j3) {
+  const int j23 = fdivmod(p, I.n_p1, I.rn_p1, j1);
+  j3 = fdivmod(j23, I.n_p2, I.rn_p2, j2);
+}
+
+// One chunk: lines ch.p0 .. ch.p1-1 of instance I in batches of I.batch
+// lines; W cells per item (W = I.w).
+//
+// Per batch: each thread issues the loads of its item (u, w, un, table
+// rows), the first nb threads put their line's moment weights in LDS, every
+// thread adds the previous batch's outputs into its moment sums and stores
+// its u into the halo-padded LDS lines; barrier; each item reconstructs the
+// W + 1 lower faces of its cells from LDS (the indicator quantities of the
+// W + 4 cells they need evaluated in the thread), forms the stage values,
+// stores them to memory and to the LDS output rows; barrier.
+//
+// Moment order: thread t sums cell t mod n1 over the chunk lines whose
+// chunk-relative index is congruent to t / n1 modulo G = 256 / n1, in
+// increasing line order; the G group sums of a cell are then added in group
+// order.  The order depends on n1 and the chunk only, not on the batch size.
+// Stage forms (out = a*un + b*(w + cscale*dt*L(u))) the kernel is
+// specialised for: w = u, a = 0, b = 1 (the first stage of either stepper);
+// w = u, a != 0 (SSP-RK3 stages 2 and 3); anything else.
+enum { ENS_FORM_FIRST = 0, ENS_FORM_COMBINE = 1, ENS_FORM_GENERAL = 2 };
+
+template <typename T, int R, int W, int FORM>
+__device__ void ens_chunk(const EnsStageArgs& ea, const EnsChunk& ch, const EnsInst& I,
+                          char* smem) {
+  const int t = threadIdx.x, n1 = I.n1, nv = n1 / W, B = I.batch;
+  const int ls = n1 + 2 * ENS_HALO;
+  T* F = (T*)smem;           // [B][ls], cell i of line l at l * ls + ENS_HALO + i
+  T* O = F + (long)B * ls;   // [B][n1]
+  double4* LW = (double4*)(smem + ens_chunk_lds_lines<T>(n1, B));  // [2][B]
+  const long base = I.off_f;
   const T* u = (const T*)ea.u + base;
-  const T* tab = (const T*)ea.tab + I.off_tab;
   const T* wv = (const T*)ea.w + base;
   const T* unv = (const T*)ea.un + base;
-  const bool w_is_u = ea.w == ea.u;
+  T* out = (T*)ea.out + base;
+  const T* tab = (const T*)ea.tab + I.off_tab;
+  const bool w_is_u = FORM == ENS_FORM_GENERAL ? ea.w == ea.u : true;
+  const bool use_un = FORM == ENS_FORM_GENERAL ? ea.use_un != 0 : FORM == ENS_FORM_COMBINE;
+  const bool b_one = FORM == ENS_FORM_GENERAL ? ea.b_one != 0 : FORM == ENS_FORM_FIRST;
   const bool src = I.source != 0, src2 = src && I.p_dim >= 2, src3 = src && I.p_dim >= 3;
-  in.C1 = T(centre(I.p1s, j1, I.dp1));
-#pragma unroll
-  for (int k = 0; k < NR; ++k) {
-    const int i = lane_in_line + 64 * k;
-    const bool live = live_line && i < n1;
-    in.f[k] = live ? __builtin_nontemporal_load(u + i) : T(0);
-    in.t1[k] = live && src ? tab[(long)j1 * n1 + i] : T(0);
-    in.t2[k] = live && src2 ? tab[(long)(I.n_p1 + j2) * n1 + i] : T(0);
-    in.t3[k] = live && src3 ? tab[(long)(I.n_p1 + I.n_p2 + j3) * n1 + i] : T(0);
-    in.wc[k] = live && !w_is_u ? __builtin_nontemporal_load(wv + i) : T(0);
-    in.unc[k] = live && ea.use_un ? __builtin_nontemporal_load(unv + i) : T(0);
-  }
-}
-
-// Stencil, source and RK combination of the q1 lines of one wave.  Lane l
-// works on cells i1 = c0 + 64 k (k < NR) of the line starting at ``line``
-// (LDS, halo-padded; p-cell p), all lanes of a line sharing C1 and the table
-// rows.
-template <typename T, int R, int NR>
-__device__ inline void ens_lines(const EnsStageArgs& ea, const EnsInst& I, int p, int lane_in_line,
-                                 bool live_line, const EnsLineIn<T, NR>& in, T* fl, T* kl, T* sl) {
-  const int n1 = I.n1;
-  const T C1 = in.C1;
-  const bool pos = C1 > T(0);
-  const long base = I.off_f + (long)p * n1;
-  const bool w_is_u = ea.w == ea.u;
-  int i1[NR];
-  bool live[NR];
-#pragma unroll
-  for (int k = 0; k < NR; ++k) {
-    i1[k] = lane_in_line + 64 * k;
-    live[k] = live_line && i1[k] < n1;
-    if (!live[k]) i1[k] = 0;
-  }
-#pragma unroll
-  for (int k = 0; k < NR; ++k) ql_put(fl, i1[k], live[k], n1, in.f[k]);
-  __builtin_amdgcn_wave_barrier();
-  T m3[NR], m2[NR], m1[NR], p1[NR], p2[NR], kap[NR];
-#pragma unroll
-  for (int k = 0; k < NR; ++k) {
-    const T* line = fl + i1[k];
-    m3[k] = line[-3];
-    m2[k] = line[-2];
-    m1[k] = line[-1];
-    p1[k] = line[1];
-    p2[k] = line[2];
-    kap[k] = ql_kappa<R, T>(m1[k], in.f[k], p1[k]);
-    ql_put(kl, i1[k], live[k], n1, kap[k]);
-  }
-  __builtin_amdgcn_wave_barrier();
-  T s[NR];
-#pragma unroll
-  for (int k = 0; k < NR; ++k) {
-    T km2 = T(0), km1 = T(0), kp1 = T(0);
-    if constexpr (R == RC_WENO5) {
-      km2 = kl[i1[k] - 2];
-      km1 = kl[i1[k] - 1];
-      kp1 = kl[i1[k] + 1];
-    }
-    // lower face of cell i: right face of i-1 (C > 0) or left face of i
-    s[k] = pos ? ql_face<R, true, T>(m3[k], m2[k], m1[k], in.f[k], p1[k], km2, km1, kap[k])
-               : ql_face<R, false, T>(m2[k], m1[k], in.f[k], p1[k], p2[k], km1, kap[k], kp1);
-    ql_put(sl, i1[k], live[k], n1, s[k]);
-  }
-  __builtin_amdgcn_wave_barrier();
   const T tinv = T(I.inv_dq), ttau = T(I.inv_tau), tcdt = T(I.cdt * ea.cscale);
   const T ta = T(ea.a), tb = T(ea.b);
-  T* out = (T*)ea.out + base;
+  const float rnv = 1.0f / float(nv);
+  const int G = ENS_THREADS / n1;
+  const int mg = t / n1, mi = t - mg * n1;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  // this thread's item: line il of the batch, cells ic .. ic + W - 1
+  int ic;
+  const int il = fdivmod(t, nv, rnv, ic);
+  ic *= W;
+
+  // moment sums of one batch (lines l0 .. l0+nb-1, outputs in O, weights in lw)
+  auto accumulate = [&](int l0, int nb, const double4* lw) {
+    if (mg >= G) return;
+    const int r0 = (l0 - ch.p0) % G;
+    for (int l = (mg - r0 + G) % G; l < nb; l += G) {
+      const double4 v = lw[l];
+      const double x = double(O[l * n1 + mi]);
+      acc[0] += x;
+      acc[1] += x * v.x;
+      acc[2] += x * v.y;
+      acc[3] += x * v.z;
+      acc[4] += x * v.w;
+    }
+  };
+
+  int k = 0, l0 = ch.p0;
+  for (; l0 < ch.p1; l0 += B, ++k) {
+    const int nb = min(B, ch.p1 - l0);
+    double4* lw = LW + (k & 1) * B;
+    const bool live = il < nb;
+    // loaded where used: u for live items, w, un and the table rows when the
+    // form and the instance read them
+    T f[W], wc[W], uc[W], t1[W], t2[W], t3[W];
+    int j1, j2, j3;
+    ens_jjj(I, l0 + (live ? il : 0), j1, j2, j3);
+    const T C = T(centre(I.p1s, j1, I.dp1));
+    const long g = (long)(l0 + il) * n1 + ic;
+    if (live) {
+      ens_ld<T, W, true>(u + g, f);
+      if (!w_is_u) ens_ld<T, W, true>(wv + g, wc);
+      if (use_un) ens_ld<T, W, true>(unv + g, uc);
+      if (src) ens_ld<T, W, false>(tab + (long)j1 * n1 + ic, t1);
+      if (src2) ens_ld<T, W, false>(tab + (long)(I.n_p1 + j2) * n1 + ic, t2);
+      if (src3) ens_ld<T, W, false>(tab + (long)(I.n_p1 + I.n_p2 + j3) * n1 + ic, t3);
+    }
+    if (t < nb) {
+      int a1, a2, a3;
+      ens_jjj(I, l0 + t, a1, a2, a3);
+      const double v1 = centre(I.p1s, a1, I.dp1), v2 = centre(I.p2s, a2, I.dp2);
+      const double v3 = centre(I.p3s, a3, I.dp3);
+      lw[t] = make_double4(v1, v2, v3, 0.5 * ((v1 * v1 + v2 * v2) + v3 * v3));
+    }
+    // the previous batch (always a full one) while this batch's loads are in flight
+    if (k > 0) accumulate(l0 - B, B, LW + ((k - 1) & 1) * B);
+    T* row = F + il * ls + ENS_HALO;
+    if (live) {
+      ens_st<T, W, false>(row + ic, f);
+      if constexpr (W >= ENS_HALO) {
+        // periodic copies: cells 0..3 after the line, cells n1-4..n1-1 before it
+        if (ic == 0) ens_st<T, ENS_HALO, false>(row + n1, f);
+        if (ic == n1 - W) ens_st<T, ENS_HALO, false>(row - ENS_HALO, f + W - ENS_HALO);
+      } else {
+        if (ic < ENS_HALO) row[n1 + ic] = f[0];
+        if (ic >= n1 - ENS_HALO) row[ic - n1] = f[0];
+      }
+    }
+    __syncthreads();
+    if (live) {
+      // y[q] = u at cell ic - 4 + q of the line
+      T y[W + 8];
+      if constexpr (W == 1) {
 #pragma unroll
-  for (int k = 0; k < NR; ++k) {
-    if (!live[k]) continue;
-    const int i = i1[k];
-    const T f = in.f[k];
-    const T su = sl[i + 1];
-    T d = -(C1 * su - C1 * s[k]) * tinv;
-    if (I.source) {
-      T fmb = in.t1[k];
-      if (I.p_dim >= 2) fmb = fmb * in.t2[k];
-      if (I.p_dim >= 3) fmb = fmb * in.t3[k];
-      d = d + (fmb - f) * ttau;
+        for (int q = 1; q < 8; ++q) y[q] = row[ic + q - 4];
+        y[0] = y[8] = T(0);
+      } else {
+        ens_ld<T, ENS_HALO, false>(row + ic - ENS_HALO, y);
+#pragma unroll
+        for (int q = 0; q < W; ++q) y[ENS_HALO + q] = f[q];
+        ens_ld<T, ENS_HALO, false>(row + ic + W, y + ENS_HALO + W);
+      }
+      // indicator quantities of cells ic - 2 + q, q < W + 4
+      T kap[W + 4];
+#pragma unroll
+      for (int q = 0; q < W + 4; ++q) kap[q] = ql_kappa<R, T>(y[q + 1], y[q + 2], y[q + 3]);
+      // lower face of cell ic + q (q = W: the upper face of the last cell):
+      // right face of the cell below (C > 0) or left face of the cell
+      T s[W + 1];
+      if (C > T(0)) {
+#pragma unroll
+        for (int q = 0; q <= W; ++q)
+          s[q] = ql_face<R, true, T>(y[q + 1], y[q + 2], y[q + 3], y[q + 4], y[q + 5], kap[q],
+                                     kap[q + 1], kap[q + 2]);
+      } else {
+#pragma unroll
+        for (int q = 0; q <= W; ++q)
+          s[q] = ql_face<R, false, T>(y[q + 2], y[q + 3], y[q + 4], y[q + 5], y[q + 6],
+                                      kap[q + 1], kap[q + 2], kap[q + 3]);
+      }
+      T o[W];
+#pragma unroll
+      for (int q = 0; q < W; ++q) {
+        const T fq = y[q + 4];
+        T d = -(C * s[q + 1] - C * s[q]) * tinv;
+        if (src) {
+          T fmb = t1[q];
+          if (src2) fmb = fmb * t2[q];
+          if (src3) fmb = fmb * t3[q];
+          d = d + (fmb - fq) * ttau;
+        }
+        const T tv = (w_is_u ? fq : wc[q]) + d * tcdt;
+        o[q] = use_un ? ta * uc[q] + tb * tv : (b_one ? tv : tb * tv);
+      }
+      ens_st<T, W, true>(out + g, o);
+      ens_st<T, W, false>(O + il * n1 + ic, o);
     }
-    const T tv = (w_is_u ? f : in.wc[k]) + d * tcdt;
-    __builtin_nontemporal_store(ea.use_un ? ta * in.unc[k] + tb * tv : (ea.b_one ? tv : tb * tv),
-                                out + i);
+    __syncthreads();
   }
-  __builtin_amdgcn_wave_barrier();
-}
-
-// (j1, j2, j3) of the p-cell ``step`` further on.
-__device__ inline void ens_advance(const EnsInst& I, int step, int& j1, int& j2, int& j3) {
-  j1 += step;
-  while (j1 >= I.n_p1) {
-    j1 -= I.n_p1;
-    if (++j2 == I.n_p2) {
-      j2 = 0;
-      ++j3;
+  // the last batch, then the group sums in group order
+  accumulate(l0 - B, ch.p1 - (l0 - B), LW + ((k - 1) & 1) * B);
+  __syncthreads();
+  double* red = (double*)smem;  // [G][5][n1]
+  if (mg < G)
+#pragma unroll
+    for (int q = 0; q < 5; ++q) red[(mg * 5 + q) * n1 + mi] = acc[q];
+  __syncthreads();
+  if (t < n1) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      double sum = 0.0;
+      for (int g2 = 0; g2 < G; ++g2) sum += red[(g2 * 5 + q) * n1 + t];
+      ea.part[I.off_part + ((long)ch.chunk * 5 + q) * n1 + t] = sum;
     }
-  }
-}
-
-// A wave task of lines longer than 64 cells: one line at a time, the next
-// line's inputs fetched while the current one is worked on.
-template <typename T, int R, int NR>
-__device__ inline void ens_task(const EnsStageArgs& ea, const EnsInst& I, const EnsTile& tile,
-                                int lane, T* fl, T* kl, T* sl) {
-  // indices of the next line to fetch
-  int j1 = tile.p0 % I.n_p1, j23 = tile.p0 / I.n_p1;
-  int j2 = j23 % I.n_p2, j3 = j23 / I.n_p2;
-  EnsLineIn<T, NR> A, B;
-  ens_fetch<T, NR>(ea, I, tile.p0, j1, j2, j3, lane, true, A);
-  ens_advance(I, 1, j1, j2, j3);
-  for (int q = 0; q < tile.np;) {
-    bool more = q + 1 < tile.np;
-    ens_fetch<T, NR>(ea, I, more ? tile.p0 + q + 1 : tile.p0, j1, j2, j3, lane, more, B);
-    ens_advance(I, 1, j1, j2, j3);
-    ens_lines<T, R, NR>(ea, I, tile.p0 + q, lane, true, A, fl, kl, sl);
-    if (++q == tile.np) break;
-    more = q + 1 < tile.np;
-    ens_fetch<T, NR>(ea, I, more ? tile.p0 + q + 1 : tile.p0, j1, j2, j3, lane, more, A);
-    ens_advance(I, 1, j1, j2, j3);
-    ens_lines<T, R, NR>(ea, I, tile.p0 + q, lane, true, B, fl, kl, sl);
-    ++q;
   }
 }
 
 #ifndef BX_NO_KERNELS
-// Persistent stage kernel: workgroup g walks the wave tasks
-// [wg_begin[g], wg_begin[g+1]) with its WPG waves taking every WPG-th task.
-template <typename T, int R>
-__global__ __launch_bounds__(256) void ens_stage_kernel(EnsStageArgs ea) {
-  __shared__ T fl[4][ENS_LDS];
-  __shared__ T kl[4][ENS_LDS];
-  __shared__ T sl[4][ENS_LDS];
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int nwave = blockDim.x >> 6;
-  const int tb = ea.wg_begin[blockIdx.x], te = ea.wg_begin[blockIdx.x + 1];
-  for (int t = tb + wave; t < te; t += nwave) {
-    const EnsTile tile = ea.tiles[t];
-    const EnsInst I = ea.inst[tile.inst];
-    if (I.n1 <= 64) {
-      // 64 / n1 lines per wave, one cell per lane; line l starts at
-      // l * (n1 + 2 halo) + halo in LDS
-      const int lines = 64 / I.n1;
-      const int line = lane / I.n1, c = lane % I.n1;
-      const int off = (line < lines ? line : 0) * (I.n1 + 2 * QL_HALO) + QL_HALO;
-      // p-cell of this lane's line and its (j1, j2, j3), carried without
-      // divisions as the task advances by ``lines`` p-cells; the two input
-      // slots alternate as in ens_task
-      const bool inline_ = line < lines;
-      const int p_first = tile.p0 + (inline_ ? line : 0);
-      int j1 = p_first % I.n_p1, j23 = p_first / I.n_p1;
-      int j2 = j23 % I.n_p2, j3 = j23 / I.n_p2;
-      T* f_ = fl[wave] + off;
-      T* k_ = kl[wave] + off;
-      T* s_ = sl[wave] + off;
-      EnsLineIn<T, 1> A, B;
-      ens_fetch<T, 1>(ea, I, p_first, j1, j2, j3, c, inline_ && line < tile.np, A);
-      ens_advance(I, lines, j1, j2, j3);
-      for (int q = 0; q < tile.np;) {
-        bool live = inline_ && q + line < tile.np;
-        bool more = inline_ && q + lines + line < tile.np;
-        ens_fetch<T, 1>(ea, I, more ? tile.p0 + q + lines + line : tile.p0, j1, j2, j3, c, more, B);
-        ens_advance(I, lines, j1, j2, j3);
-        ens_lines<T, R, 1>(ea, I, live ? tile.p0 + q + line : tile.p0, c, live, A, f_, k_, s_);
-        q += lines;
-        if (q >= tile.np) break;
-        live = inline_ && q + line < tile.np;
-        more = inline_ && q + lines + line < tile.np;
-        ens_fetch<T, 1>(ea, I, more ? tile.p0 + q + lines + line : tile.p0, j1, j2, j3, c, more, A);
-        ens_advance(I, lines, j1, j2, j3);
-        ens_lines<T, R, 1>(ea, I, live ? tile.p0 + q + line : tile.p0, c, live, B, f_, k_, s_);
-        q += lines;
-      }
-    } else if (I.n1 <= 128) {
-      ens_task<T, R, 2>(ea, I, tile, lane, fl[wave] + QL_HALO, kl[wave] + QL_HALO,
-                        sl[wave] + QL_HALO);
-    } else {
-      ens_task<T, R, 4>(ea, I, tile, lane, fl[wave] + QL_HALO, kl[wave] + QL_HALO,
-                        sl[wave] + QL_HALO);
-    }
-  }
+template <typename T, int R, int FORM>
+__global__ __launch_bounds__(ENS_THREADS) void ens_chunk_kernel(EnsStageArgs ea) {
+  extern __shared__ __align__(16) char ens_smem[];
+  const EnsChunk ch = ea.chunks[blockIdx.x];
+  const EnsInst I = ea.inst[ch.inst];
+  if (I.w > 1) ens_chunk<T, R, 32 / sizeof(T), FORM>(ea, ch, I, ens_smem);
+  else ens_chunk<T, R, 1, FORM>(ea, ch, I, ens_smem);
 }
 
 // Partial moments of one (instance, p-chunk): the chunk is a contiguous

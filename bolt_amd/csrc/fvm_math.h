@@ -291,19 +291,20 @@ __host__ __device__ inline void write_moment_block(double* mom, int Q, int qc, c
 
 // Sum of the five moment partials of one q-cell over nchunk chunks,
 // part[(c * 5 + k) * stride], in chunk order for each k (the same result as
-// the plain loop); loads are issued four chunks at a time.
+// the plain loop).
 __host__ __device__ inline void sum_moment_partials(const double* part, int nchunk, long stride,
                                                     double m[5]) {
   double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   int c = 0;
-  for (; c + 4 <= nchunk; c += 4) {
-    double v[4][5];
+  // eight chunks' loads in flight at a time; the sums still run in chunk order
+  for (; c + 8 <= nchunk; c += 8) {
+    double v[8][5];
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < 8; ++j)
 #pragma unroll
       for (int k = 0; k < 5; ++k) v[j][k] = part[((long)(c + j) * 5 + k) * stride];
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < 8; ++j)
 #pragma unroll
       for (int k = 0; k < 5; ++k) s[k] += v[j][k];
   }

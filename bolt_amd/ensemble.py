@@ -5,15 +5,17 @@ timestep.py:101-160); many small problems then cost one launch sequence each.
 ``Ensemble`` takes already built ``nonlinear_solver`` objects of uneven sizes
 (periodic q1, N_q2 = 1, fields off, the same reconstruction and precision)
 with a time step each, pools their states in the q1-line layout and advances
-all of them with ``boltx_ensemble_step``: per RK stage one persistent launch
-over a host-balanced work list, one batched moment launch and one batched
-moment-block / Maxwellian-table launch.
+all of them with ``boltx_ensemble_step``: per RK stage one launch of a workgroup per
+(member, chunk of whole q1 lines) that forms the stage and the FP64 partial
+moments of its output, and one batched moment-block / Maxwellian-table
+launch.
 
-The work-list shape (waves per workgroup, cells per wave task, persistent
-workgroups per CU, cells per moment workgroup) is picked by ``autotune``,
-which times candidate shapes on a scratch copy of the ensemble and caches the
-winner per ensemble signature, in the process and in a JSON file (per GPU
-model) that later processes read instead of tuning again.
+The work shape (cells per chunk, cells a workgroup stages in LDS at once) is
+picked by ``autotune``, which times candidate shapes on a scratch copy of the
+ensemble and caches the winner per ensemble signature, in the process and in
+a JSON file (per GPU model) that later processes read instead of tuning
+again.  The chunk size groups the moment sums, so it changes results in the
+last bits; the batch size does not.
 """
 import ctypes
 import itertools
@@ -27,7 +29,8 @@ from . import _lib
 
 F64 = torch.float64
 
-DEFAULT_TUNING = (4, 1024, 4, 16384)
+DEFAULT_TUNING = (16384, 2048)
+TUNING_FIELDS = len(_lib.EnsTuning._fields_)
 _TUNED = {}
 # Tunings persist across processes in a JSON file (one entry per ensemble
 # signature and GPU model), by default under the user's cache directory.
@@ -48,7 +51,7 @@ def load_tuning(path, key):
         with open(path) as fh:
             table = json.load(fh)
         t = table[key]
-        if isinstance(t, list) and len(t) == 4 and all(isinstance(x, int) for x in t):
+        if isinstance(t, list) and len(t) == TUNING_FIELDS and all(isinstance(x, int) for x in t):
             return tuple(t)
     except (OSError, ValueError, KeyError, TypeError):
         pass
@@ -232,11 +235,13 @@ class Ensemble:
 
     # -- autotuner -----------------------------------------------------------
     def autotune(self, candidates=None, steps=2):
-        """Time each candidate work-list shape over ``steps`` steps on a scratch
-        copy of the ensemble and keep the fastest; the result is cached for
-        ensembles with the same signature.  Returns the chosen tuning."""
+        """Time each candidate work shape (chunk_cells, batch_cells) over
+        ``steps`` steps on a scratch copy of the ensemble and keep the
+        fastest; the result is cached for ensembles with the same signature.
+        Returns the chosen tuning."""
         if candidates is None:
-            candidates = list(itertools.product((1, 2, 4), (256, 1024, 4096), (2, 8), (16384,)))
+            candidates = list(itertools.product((4096, 8192, 16384, 32768, 65536),
+                                                (1024, 2048)))
         saved_f = self.f
         best, best_t = None, float('inf')
         for cand in candidates:
@@ -252,20 +257,7 @@ class Ensemble:
             torch.cuda.synchronize()
             t = time.perf_counter() - t0
             if t < best_t:
-                best, best_t = cand, t
-        # refine the moment chunking with the best shape
-        for chunk in (4096, 32768, 65536, 131072):
-            cand = best[:3] + (chunk,)
-            self.f = saved_f.clone()
-            self._set_tuning(cand)
-            self.step(1)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            self.step(steps)
-            torch.cuda.synchronize()
-            t = time.perf_counter() - t0
-            if t < best_t:
-                best, best_t = cand, t
+                best, best_t = tuple(cand), t
         self.f = saved_f
         self.steps_taken = 0
         _TUNED[self.signature()] = best

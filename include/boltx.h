@@ -267,12 +267,12 @@ size_t boltx_adjoint_workspace_size(const boltx_grid *grid, const boltx_physics 
  * layout [P][n_q1] inside pooled device arrays (member i at the element
  * offset boltx_ensemble_layout() reports); all members share the
  * reconstruction and the precision.  Each stage of the whole ensemble is one
- * persistent launch over a host-built work list (wave tasks of about
- * cells_per_task cells, contiguous ranges of equal cost for
- * groups_per_cu * #CU workgroups of waves_per_group waves), then one batched
- * moment launch and one batched moment-block / Maxwellian-table launch.  The
- * reference runs such ensembles one instance at a time: this replaces that
- * loop over timestep_fvm (fvm_operator.py:6-50) for many small problems. */
+ * launch of a workgroup per (member, chunk of chunk_cells cells in whole
+ * q1 lines) that forms the stage and the FP64 partial moments of its output,
+ * then one batched moment-block / Maxwellian-table launch.  The pooled
+ * arrays and the workspace must be 16-byte aligned.  The reference runs such
+ * ensembles one instance at a time: this replaces that loop over
+ * timestep_fvm (fvm_operator.py:6-50) for many small problems. */
 enum boltx_stepper { BOLTX_RK2 = 0, BOLTX_SSP_RK3 = 1 };
 
 typedef struct boltx_ens_member {
@@ -282,10 +282,10 @@ typedef struct boltx_ens_member {
 } boltx_ens_member;
 
 typedef struct boltx_ens_tuning {
-  int32_t waves_per_group; /* 1, 2 or 4 */
-  int32_t cells_per_task;  /* target phase-space cells per wave task */
-  int32_t groups_per_cu;   /* persistent workgroups per compute unit, 1..64 */
-  int32_t chunk_cells;     /* target cells per moment workgroup */
+  int32_t chunk_cells; /* target cells per chunk (rounded down to whole q1 lines, at least one);
+                          the moment sums are grouped by chunk, so results depend on it */
+  int32_t batch_cells; /* target cells a chunk workgroup stages in LDS at once (whole lines);
+                          results do not depend on it */
 } boltx_ens_tuning;
 
 /* Element offsets of the members in the pooled f-sized arrays (offsets[n],

@@ -27,17 +27,21 @@ def test_layout_offsets_pool_members_in_order():
     assert list(offs) == want and total == o
 
 
-def test_workspace_grows_with_tasks_and_checks_input():
+def test_workspace_grows_with_chunks_and_checks_input():
+    """Smaller chunks mean more partial-moment blocks in the workspace; the
+    batch size does not change the workspace; non-positive sizes are
+    rejected."""
     L = _lib.load()
     arr = (_lib.EnsMember * 2)(_member(128, 48), _member(40, 24))
-    small = _lib.EnsTuning(4, 4096, 1, 16384)
-    big = _lib.EnsTuning(4, 64, 8, 1024)
+    small = _lib.EnsTuning(16384, 2048)
+    big = _lib.EnsTuning(1024, 2048)
     a = L.boltx_ensemble_workspace_size(2, arr, ctypes.byref(small))
     b = L.boltx_ensemble_workspace_size(2, arr, ctypes.byref(big))
     assert 0 < a < b
-    bad = _lib.EnsTuning(3, 1024, 4, 16384)
-    assert L.boltx_ensemble_workspace_size(2, arr, ctypes.byref(bad)) == 0
-    assert b"waves_per_group" in L.boltx_last_error()
+    assert L.boltx_ensemble_workspace_size(2, arr, ctypes.byref(_lib.EnsTuning(16384, 64))) == a
+    for bad in (_lib.EnsTuning(0, 2048), _lib.EnsTuning(16384, -1)):
+        assert L.boltx_ensemble_workspace_size(2, arr, ctypes.byref(bad)) == 0
+        assert b"chunk_cells" in L.boltx_last_error()
     mixed = (_lib.EnsMember * 2)(_member(128, 48, prec=0), _member(64, 24, prec=1))
     assert L.boltx_ensemble_workspace_size(2, mixed, ctypes.byref(small)) == 0
     assert b"precision" in L.boltx_last_error()
@@ -56,16 +60,16 @@ def test_tuning_cache_file(tmp_path):
     k1 = E.cache_key(((32, 8, 8, 1), (64, 6, 6, 1), 1, 1, 3), "AMD Instinct MI355X")
     k2 = E.cache_key(((32, 8, 8, 1),), "AMD Instinct MI355X")
     assert E.load_tuning(path, k1) is None
-    assert E.store_tuning(path, k1, (4, 1024, 8, 65536))
-    assert E.store_tuning(path, k2, (2, 256, 2, 16384))
-    assert E.load_tuning(path, k1) == (4, 1024, 8, 65536)
-    assert E.load_tuning(path, k2) == (2, 256, 2, 16384)
+    assert E.store_tuning(path, k1, (65536, 1024))
+    assert E.store_tuning(path, k2, (16384, 2048))
+    assert E.load_tuning(path, k1) == (65536, 1024)
+    assert E.load_tuning(path, k2) == (16384, 2048)
     assert E.load_tuning(path, E.cache_key(((32, 8, 8, 1),), "other GPU")) is None
     with open(path, "w") as fh:
-        fh.write('{"%s": [1, 2]}' % k1.replace('"', '\\"'))
+        fh.write('{"%s": [1, 2, 3, 4]}' % k1.replace('"', '\\"'))
     assert E.load_tuning(path, k1) is None
     with open(path, "w") as fh:
         fh.write("not json")
     assert E.load_tuning(path, k1) is None
-    assert E.store_tuning(path, k1, (1, 256, 2, 4096))
-    assert E.load_tuning(path, k1) == (1, 256, 2, 4096)
+    assert E.store_tuning(path, k1, (4096, 1024))
+    assert E.load_tuning(path, k1) == (4096, 1024)

@@ -89,15 +89,24 @@ def test_ensemble_fp32_bulk_vs_fp64(gpu):
         assert l2 <= 1e-5, (i, l2)
 
 
-def test_work_list_shape_does_not_change_results(gpu):
+def test_batch_size_does_not_change_results(gpu):
+    """The cells a workgroup stages at once (one line per batch up to whole
+    chunks, with a partial last batch) leave the results bit-identical; a
+    different chunk size regroups only the FP64 moment sums."""
     from bolt_amd.ensemble import Ensemble
     cases = _members("fp64", SIZES, seed=5)
     outs = []
-    for tuning in [(1, 256, 2, 16384), (4, 4096, 8, 16384), (2, 1024, 1, 16384)]:
+    for tuning in [(16384, 256), (16384, 2048), (16384, 1024), (16384, 1)]:
         ens = Ensemble([hip_solver(c) for c in cases], [c.dt for c in cases], tuning=tuning)
         ens.step(3)
         outs.append(ens.f.clone())
-    assert torch.equal(outs[0], outs[1]) and torch.equal(outs[0], outs[2])
+    for o in outs[1:]:
+        assert torch.equal(outs[0], o)
+    ens = Ensemble([hip_solver(c) for c in cases], [c.dt for c in cases], tuning=(512, 2048))
+    ens.step(3)
+    d = _rel_linf(ens.f, outs[0])
+    print(f"chunk 512 vs 16384 cells after 3 steps: rel Linf {d:.3e}", flush=True)
+    assert d <= 1e-10
 
 
 def test_autotune_keeps_state_and_picks_a_candidate(gpu):
@@ -106,9 +115,9 @@ def test_autotune_keeps_state_and_picks_a_candidate(gpu):
     ens = Ensemble([hip_solver(c) for c, _ in members], [dt for _, dt in members],
                    autotune=False)
     f0 = ens.f.clone()
-    cands = [(1, 512, 2, 16384), (4, 2048, 4, 16384)]
+    cands = [(8192, 1024), (16384, 2048)]
     best = ens.autotune(candidates=cands, steps=1)
-    assert best[:3] in [c[:3] for c in cands]
+    assert best in cands
     ens._set_tuning(best)
     assert torch.equal(ens.f, f0)
     ens.step(2)

@@ -85,7 +85,7 @@ def test_adjoint_and_ensemble_reject_bad_input():
     m[0].grid.n_q2 = 1
     m[0].phys = ph
     m[0].dt = 0.0
-    t = _lib.EnsTuning(4, 1024, 4, 16384)
+    t = _lib.EnsTuning(16384, 2048)
     assert lib.boltx_ensemble_step(1, m, C.byref(t), 1, None, None, None, None, 1, None) == -1
     assert b"dt" in lib.boltx_last_error()
 
