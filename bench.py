@@ -30,6 +30,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# work shape of the ensemble traffic probe (the autotuner's pick for the
+# 96-member workload on MI355X)
+ENSEMBLE_PROBE_TUNING = (32768, 2048)
 METRIC = ("phase-space cell-updates/s, FP32-bulk vs FP64 at rel-L2≤1e-5; "
           "% HBM peak")
 
@@ -110,15 +113,16 @@ def run_gpu(precision, steps, warmup):
     }
 
 
-def _traffic(key):
-    """HBM bytes per launch of the workload's dominant kernel measured with
-    rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel
-    calibration), as ``bench.py --traffic`` wrote them into
+def _traffic(key, field="bytes_per_launch"):
+    """HBM bytes per launch of the workload's dominant kernel (or, with
+    ``field="bytes_per_sequence"``, summed over the launches of one step)
+    measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by a
+    copy-kernel calibration), as ``bench.py --traffic`` wrote them into
     profiles/traffic.json; None if not measured."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as fh:
-            return json.load(fh).get(key, {}).get("bytes_per_launch")
+            return json.load(fh).get(key, {}).get(field)
     except (OSError, ValueError):
         return None
 
@@ -136,10 +140,11 @@ def _pmc_rows(path, counter):
     return [(k, v) for _, k, v in rows]
 
 
-def traffic_from_pmc(key, fetch_csv, write_csv, kernel, algorithmic):
+def traffic_from_pmc(key, fetch_csv, write_csv, kernel, algorithmic, after=None, extra=None):
     """HBM bytes per launch of ``kernel`` (names separated by '|') from one
     FETCH_SIZE run and one WRITE_SIZE run of ``bench.py --probe`` (KiB per
-    dispatch).  Each counter
+    dispatch; with ``after``, only the dispatches from the first one whose
+    name holds ``after`` on).  Each counter
     is scaled by what it reads for the probe's first 256 MiB device copy
     (read 256 MiB, write 256 MiB) in the same run.  ``algorithmic`` are the
     algorithmic bytes of each launch, in order.  Updates
@@ -149,6 +154,9 @@ def traffic_from_pmc(key, fetch_csv, write_csv, kernel, algorithmic):
     for name, path in (("fetch", fetch_csv), ("write", write_csv)):
         rows = _pmc_rows(path, name.upper() + "_SIZE")
         cal = [v for k, v in rows if "copy" in k.lower()]
+        if after:
+            first = next((i for i, (k, _) in enumerate(rows) if after in k), len(rows))
+            rows = rows[first:]
         launches = [v for k, v in rows if any(n in k for n in kernel.split("|"))]
         if not cal or not launches:
             raise SystemExit("no copy or %s dispatches in %s" % (kernel, path))
@@ -159,8 +167,8 @@ def traffic_from_pmc(key, fetch_csv, write_csv, kernel, algorithmic):
                      out["write_kib_per_launch"][i] * out["write_scale"]) for i in range(n)]
     entry = dict(out, kernel=kernel, per_launch_bytes=per,
                  algorithmic_per_launch_bytes=list(algorithmic[:n]),
-                 bytes_per_launch=sum(per) / n,
-                 ratio_to_algorithmic=sum(per) / sum(algorithmic[:n]))
+                 bytes_per_launch=sum(per) / n, bytes_per_sequence=sum(per),
+                 ratio_to_algorithmic=sum(per) / sum(algorithmic[:n]), **(extra or {}))
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as fh:
@@ -358,7 +366,11 @@ def run_ensemble(steps=200, sample_steps=10):
             "ms_per_step_without_graph": t_eager / steps * 1e3,
             "roofline": {"bound": "hbm", "level": "step (all launches of a step)",
                          "achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": step_gbs / HBM_PEAK_GBS, "traffic": None},
+                         "frac": step_gbs / HBM_PEAK_GBS,
+                         # counter-measured HBM bytes of one step (stage and
+                         # finalize launches) at the probe's work shape
+                         "traffic": _traffic("ensemble", "bytes_per_sequence"),
+                         "traffic_tuning": list(ENSEMBLE_PROBE_TUNING)},
             "tuning": {"chunk_cells": ens.tuning[0], "batch_cells": ens.tuning[1]},
             "autotune_s": t_tune,
             "sequential_fp64": {"value": seq, "sample": "%d steps per member, one member "
@@ -419,12 +431,23 @@ def probe(which):
     and their moment passes) or of the 2D3V electromagnetic one (``em-fp64``,
     ``em-fp32-bulk``: three qcell_kernel stages)."""
     import torch
-    from tests.cases import case_2d2v, case_2d3v, hip_solver
+    from tests.cases import case_2d2v, case_2d3v, case_c5, hip_solver
     x = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")  # 256 MiB
     y = torch.empty_like(x)
     for _ in range(3):
         y.copy_(x)
     torch.cuda.synchronize()
+    if which == "ensemble":
+        # one SSP-RK3 step of the 96-member workload at the work shape the
+        # autotuner picks for it on this GPU
+        from bolt_amd.ensemble import Ensemble
+        m = case_c5(seed=0, precision="fp32-bulk")
+        ens = Ensemble([hip_solver(c) for c, _ in m], [dt for _, dt in m],
+                       tuning=ENSEMBLE_PROBE_TUNING, autotune=False, tuning_cache=False)
+        ens.step(1)
+        torch.cuda.synchronize()
+        print("probe done", flush=True)
+        return
     if which.startswith("em-"):
         c = case_2d3v(seed=0, precision=which[3:])
     else:
@@ -437,7 +460,13 @@ def probe(which):
 
 def probe_algorithmic(which):
     """Algorithmic bytes of the probe's three stage launches: stage 1 reads
-    u and writes out, stages 2 and 3 also read un."""
+    u and writes out, stages 2 and 3 also read un.  The ensemble's step
+    alternates stage and finalize launches; the finalize has no algorithmic
+    bytes of its own (its traffic is the moment partials and tables)."""
+    if which == "ensemble":
+        from tests.cases import case_c5
+        cells = sum(c.domain.N_q1 * c.domain.N_p1 * c.domain.N_p2 for c, _ in case_c5(seed=0))
+        return [8 * cells, 0, 12 * cells, 0, 12 * cells, 0]
     cells = 64 * 64 * 24 ** 3 if which.startswith("em-") else 128 * 128 * 48 * 48
     esz = 4 if which.endswith("fp32-bulk") else 8
     return [2 * cells * esz, 3 * cells * esz, 3 * cells * esz]
@@ -453,19 +482,27 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the 2D3V electromagnetic and adjoint measurements")
-    ap.add_argument("--probe", choices=["fp64", "fp32-bulk", "em-fp64", "em-fp32-bulk"],
+    ap.add_argument("--probe", choices=["fp64", "fp32-bulk", "em-fp64", "em-fp32-bulk", "ensemble"],
                     help="traffic probe for rocprofv3 --pmc: a 256 MiB device copy "
                          "(the counter calibration) and then one step of the workload")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="with --probe-of: turn the counter CSVs of a FETCH_SIZE and a "
                          "WRITE_SIZE run of that probe into profiles/traffic.json")
-    ap.add_argument("--probe-of", choices=["fp64", "fp32-bulk", "em-fp64", "em-fp32-bulk"])
+    ap.add_argument("--probe-of", choices=["fp64", "fp32-bulk", "em-fp64", "em-fp32-bulk",
+                                           "ensemble"])
     ap.add_argument("--only", choices=["em_2d3v", "adjoint", "ensemble"],
                     help="run one of the other workloads alone (for profiling) and print "
                          "its JSON object")
     args = ap.parse_args()
     if args.traffic:
         w = args.probe_of
+        if w == "ensemble":
+            e = traffic_from_pmc("ensemble", args.traffic[0], args.traffic[1],
+                                 "ens_chunk_kernel|ens_finalize_kernel", probe_algorithmic(w),
+                                 after="ens_chunk_kernel",
+                                 extra={"tuning": list(ENSEMBLE_PROBE_TUNING)})
+            print(json.dumps({w: e}, indent=1), flush=True)
+            return
         kernel = "qcell_kernel|qsweep_kernel" if w.startswith("em-") else "qline_kernel"
         key = "em_" + w[3:] if w.startswith("em-") else w
         e = traffic_from_pmc(key, args.traffic[0], args.traffic[1], kernel, probe_algorithmic(w))

@@ -21,10 +21,12 @@ namespace {
 struct Plan {
   std::vector<EnsInst> inst;
   std::vector<EnsChunk> chunks;  // in launch order: largest first
+  std::vector<int2> fin_blocks;  // finalize workgroups: (instance, first q1 cell)
   long pooled = 0, tab = 0, mom = 0, part = 0;
   size_t lds = 0;                // dynamic LDS bytes of the chunk kernel
   // byte offsets inside the workspace
-  size_t o_inst = 0, o_chunks = 0, o_mom = 0, o_part = 0, o_tab = 0, o_flag = 0, total = 0;
+  size_t o_inst = 0, o_chunks = 0, o_fin = 0, o_mom = 0, o_part = 0, o_tab = 0, o_flag = 0,
+         total = 0;
 };
 
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
@@ -96,6 +98,7 @@ Plan make_plan(int n, const boltx_ens_member* m, const boltx_ens_tuning* t) {
     pl.part += (long)I.nchunk * 5 * I.n1;
     for (int c = 0; c < I.nchunk; ++c)
       pl.chunks.push_back({i, c, c * lc, std::min(I.P, (c + 1) * lc)});
+    for (int q0 = 0; q0 < I.n1; q0 += ENS_FIN_CELLS) pl.fin_blocks.push_back(make_int2(i, q0));
     const size_t lds = esz == 8 ? ens_chunk_lds<double>(I.n1, I.batch)
                                 : ens_chunk_lds<float>(I.n1, I.batch);
     pl.lds = std::max(pl.lds, lds);
@@ -122,6 +125,8 @@ Plan make_plan(int n, const boltx_ens_member* m, const boltx_ens_tuning* t) {
   o = align256(o + pl.inst.size() * sizeof(EnsInst));
   pl.o_chunks = o;
   o = align256(o + pl.chunks.size() * sizeof(EnsChunk));
+  pl.o_fin = o;
+  o = align256(o + pl.fin_blocks.size() * sizeof(int2));
   pl.o_mom = o;
   o = align256(o + (size_t)pl.mom * sizeof(double));
   pl.o_part = o;
@@ -144,8 +149,9 @@ int check_aligned(const void* p, const char* what) {
 
 template <typename T>
 int finalize(const Plan& pl, char* ws, hipStream_t s) {
-  hipLaunchKernelGGL(ens_finalize_kernel<T>, dim3((unsigned)pl.inst.size(), ENS_TGROUPS), dim3(256), 0, s,
-                     (const EnsInst*)(ws + pl.o_inst), (const double*)(ws + pl.o_part),
+  hipLaunchKernelGGL(ens_finalize_kernel<T>, dim3((unsigned)pl.fin_blocks.size()),
+                     dim3(ENS_FIN_CELLS * ENS_FIN_SUBS), 0, s, (const EnsInst*)(ws + pl.o_inst),
+                     (const int2*)(ws + pl.o_fin), (const double*)(ws + pl.o_part),
                      (double*)(ws + pl.o_mom), (T*)(ws + pl.o_tab), (int*)(ws + pl.o_flag));
   return launch_status("ens_finalize_kernel");
 }
@@ -272,7 +278,8 @@ extern "C" int boltx_ensemble_init(int n, const boltx_ens_member* members,
     const void* src;
     size_t bytes;
   } pieces[] = {{pl.o_inst, pl.inst.data(), pl.inst.size() * sizeof(EnsInst)},
-                {pl.o_chunks, pl.chunks.data(), pl.chunks.size() * sizeof(EnsChunk)}};
+                {pl.o_chunks, pl.chunks.data(), pl.chunks.size() * sizeof(EnsChunk)},
+                {pl.o_fin, pl.fin_blocks.data(), pl.fin_blocks.size() * sizeof(int2)}};
   for (const Piece& pc : pieces)
     if (hipMemcpyAsync(ws + pc.off, pc.src, pc.bytes, hipMemcpyHostToDevice, s) != hipSuccess)
       return fail(BOLTX_ELAUNCH, "ensemble: copy of the work lists failed");

@@ -13,9 +13,9 @@
 //    collision_operator.py:70-128, fvm_operator.py:22-33) and sums the FP64
 //    partial moments of the stage output over the chunk's lines in a fixed
 //    order (compute_moments.py:7-59), so f is not read a second time;
-//  * ens_finalize_kernel: one workgroup per instance and table group: the
-//    moment block (moments.py:5-22, collision_operator.py:102-116) from the
-//    partials, and the separable Maxwellian tables the next stage reads.
+//  * ens_finalize_kernel: one workgroup per (instance, block of q1 cells):
+//    the moment block (moments.py:5-22, collision_operator.py:102-116) from
+//    the partials, and the separable Maxwellian tables the next stage reads.
 // ens_moments_kernel computes the same partials from a given state (the
 // initial one).  The chunk list is built on the host; no kernel waits on
 // another workgroup.
@@ -408,40 +408,81 @@ __global__ __launch_bounds__(256) void ens_moments_kernel(const EnsInst* inst,
     for (int k = 0; k < 5; ++k) part[I.off_part + ((long)ch.chunk * 5 + k) * n1 + t] = cell[k][t];
 }
 
-// Moment block and Maxwellian tables of every instance: workgroup (instance
-// blockIdx.x, table group blockIdx.y of ENS_TGROUPS), one thread per q1 cell.
-// Every group sums the partials of its q-cells (fixed order); group 0 writes
-// the moment block and the divergence flag; group g writes the table rows
-// j = g, g + ENS_TGROUPS, ...
-constexpr int ENS_TGROUPS = 8;
+// Moment block and Maxwellian tables of every instance: one workgroup per
+// (instance, block of ENS_FIN_CELLS q1 cells), so every partial is read
+// once.  Thread t works on cell c0 + (t mod ENS_FIN_CELLS) with sub-index
+// s = t / ENS_FIN_CELLS < ENS_FIN_SUBS: it sums the partials of chunks
+// s, s + ENS_FIN_SUBS, ... in increasing order, and the sub-sums of a cell
+// are added in sub-index order (an order fixed by the chunk count).  The
+// first sub writes the cell's moment block and the divergence flag; every
+// sub then writes the table rows j = s, s + ENS_FIN_SUBS, ... of the cell.
+constexpr int ENS_FIN_CELLS = 16;
+constexpr int ENS_FIN_SUBS = 16;
 
 template <typename T>
-__global__ __launch_bounds__(256) void ens_finalize_kernel(const EnsInst* inst,
-                                                           const double* part, double* mom,
-                                                           T* tab, int* flag) {
-  const EnsInst I = inst[blockIdx.x];
-  const int q = threadIdx.x;
-  if (q >= I.n1) return;
-  const int n1 = I.n1;
-  double m[5];
-  sum_moment_partials(part + I.off_part + q, I.nchunk, n1, m);
-  for (int k = 0; k < 5; ++k) m[k] *= I.measure;
-  double* mb = mom + I.off_mom;
-  if (blockIdx.y == 0) {
-    if (!isfinite(m[0])) *flag = 1;
-    write_moment_block(mb, n1, q, m, I.mass, I.kB, I.p_dim);
+__global__ __launch_bounds__(ENS_FIN_CELLS * ENS_FIN_SUBS) void ens_finalize_kernel(
+    const EnsInst* inst, const int2* blocks, const double* part, double* mom, T* tab, int* flag) {
+  __shared__ double red[5][ENS_FIN_SUBS][ENS_FIN_CELLS];
+  __shared__ double cell[5][ENS_FIN_CELLS];  // prefactor, u1, u2, u3, T
+  const int2 blk = blocks[blockIdx.x];       // (instance, first cell)
+  const EnsInst I = inst[blk.x];
+  const int n1 = I.n1, t = threadIdx.x;
+  const int sub = t / ENS_FIN_CELLS, ci = t - sub * ENS_FIN_CELLS, q = blk.y + ci;
+  const bool on = q < n1;
+  double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (on) {
+    const double* pp = part + I.off_part + q;
+    const long step = (long)ENS_FIN_SUBS * 5 * n1;
+    int c = sub;
+    // two chunks' loads in flight at a time, summed in chunk order
+    for (; c + ENS_FIN_SUBS < I.nchunk; c += 2 * ENS_FIN_SUBS) {
+      double v[2][5];
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int k = 0; k < 5; ++k) v[j][k] = pp[(long)c * 5 * n1 + j * step + (long)k * n1];
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int k = 0; k < 5; ++k) a[k] += v[j][k];
+    }
+    if (c < I.nchunk)
+#pragma unroll
+      for (int k = 0; k < 5; ++k) a[k] += pp[((long)c * 5 + k) * n1];
   }
-  if (!I.source) return;
-  const BgkCell c = bgk_cell(m[0], m[1], m[2], m[3], m[4], I.mass, I.p_dim);
-  const double pref = maxwell_prefactor(c, I.mass, I.kB, I.p_dim);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) red[k][sub][ci] = a[k];
+  __syncthreads();
+  if (sub == 0 && on) {
+    double m[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      double sum = 0.0;
+#pragma unroll
+      for (int s2 = 0; s2 < ENS_FIN_SUBS; ++s2) sum += red[k][s2][ci];
+      m[k] = sum * I.measure;
+    }
+    if (!isfinite(m[0])) *flag = 1;
+    write_moment_block(mom + I.off_mom, n1, q, m, I.mass, I.kB, I.p_dim);
+    const BgkCell bc = bgk_cell(m[0], m[1], m[2], m[3], m[4], I.mass, I.p_dim);
+    cell[0][ci] = maxwell_prefactor(bc, I.mass, I.kB, I.p_dim);
+    cell[1][ci] = bc.u1;
+    cell[2][ci] = bc.u2;
+    cell[3][ci] = bc.u3;
+    cell[4][ci] = bc.T;
+  }
+  __syncthreads();
+  if (!I.source || !on) return;
+  const double pref = cell[0][ci], u1 = cell[1][ci], u2 = cell[2][ci], u3 = cell[3][ci];
+  const double Tc = cell[4][ci];
   T* tb = tab + I.off_tab;
   const int nj = I.n_p1 + I.n_p2 + I.n_p3;
-  for (int j = blockIdx.y; j < nj; j += ENS_TGROUPS) {
+  for (int j = sub; j < nj; j += ENS_FIN_SUBS) {
     double val;
-    if (j < I.n_p1) val = pref * maxwell_factor(centre(I.p1s, j, I.dp1), c.u1, c.T, I.mass, I.kB);
+    if (j < I.n_p1) val = pref * maxwell_factor(centre(I.p1s, j, I.dp1), u1, Tc, I.mass, I.kB);
     else if (j < I.n_p1 + I.n_p2)
-      val = maxwell_factor(centre(I.p2s, j - I.n_p1, I.dp2), c.u2, c.T, I.mass, I.kB);
-    else val = maxwell_factor(centre(I.p3s, j - I.n_p1 - I.n_p2, I.dp3), c.u3, c.T, I.mass, I.kB);
+      val = maxwell_factor(centre(I.p2s, j - I.n_p1, I.dp2), u2, Tc, I.mass, I.kB);
+    else val = maxwell_factor(centre(I.p3s, j - I.n_p1 - I.n_p2, I.dp3), u3, Tc, I.mass, I.kB);
     tb[(long)j * n1 + q] = T(val);
   }
 }
