@@ -72,7 +72,8 @@ class Adjoint(ctypes.Structure):
     _fields_ = [("u", ctypes.c_void_p), ("mom_u", ctypes.c_void_p), ("lam", ctypes.c_void_p),
                 ("base", ctypes.c_void_p), ("out", ctypes.c_void_p),
                 ("a", ctypes.c_double), ("b", ctypes.c_double), ("c", ctypes.c_double),
-                ("tau_grad", ctypes.c_void_p), ("workspace", ctypes.c_void_p)]
+                ("tau_grad", ctypes.c_void_p), ("workspace", ctypes.c_void_p),
+                ("base2", ctypes.c_void_p), ("a2", ctypes.c_double)]
 
 
 class EnsMember(ctypes.Structure):

@@ -84,25 +84,25 @@ def _step_recorded(s, f, dt):
     return out, [(f, m0), (u1, m1), (u2, m2)]
 
 
-def _adjoint_stage(s, u, mom, lam, base, a, b, c, ws, tau_q):
+def _adjoint_stage(s, u, mom, lam, base, a, b, c, ws, tau_q, base2=None, a2=0.0):
     out = torch.empty_like(lam)
     adj = _lib.Adjoint(_lib.ptr(u), _lib.ptr(mom), _lib.ptr(lam), _lib.ptr(base), _lib.ptr(out),
-                       float(a), float(b), float(c), _lib.ptr(tau_q), _lib.ptr(ws))
+                       float(a), float(b), float(c), _lib.ptr(tau_q), _lib.ptr(ws),
+                       _lib.ptr(base2), float(a2))
     _lib.check(s._L.boltx_adjoint_stage(ctypes.byref(s._grid), ctypes.byref(s._phys),
                                         ctypes.byref(adj), _lib.stream_handle()))
     return out
 
 
-def _adjoint_step(s, tape, lam, dt, ws, tau_q, tau_acc):
+def _adjoint_step(s, tape, lam, dt, ws, tau_rows):
     """Reverse one step: lam (adjoint of the step's output) -> adjoint of its
-    input; adds the step's d/d tau into tau_acc."""
+    input; each stage's d/d tau per q-cell goes into the next row that
+    ``tau_rows()`` hands out."""
     src = bool(s._source_on)
 
-    def run(u, mom, lam_, base, a, b, c):
-        out = _adjoint_stage(s, u, mom, lam_, base, a, b, c, ws, tau_q if src else None)
-        if src:
-            tau_acc.add_(tau_q.sum())
-        return out
+    def run(u, mom, lam_, base, a, b, c, base2=None, a2=0.0):
+        return _adjoint_stage(s, u, mom, lam_, base, a, b, c, ws,
+                              tau_rows() if src else None, base2, a2)
 
     if s._stepper == 'RK2':
         (x0, m0), (xh, mh) = tape
@@ -114,9 +114,33 @@ def _adjoint_step(s, tape, lam, dt, ws, tau_q, tau_acc):
     l2 = run(x2, m2, lam, None, 0.0, 2 / 3, 2 / 3 * dt)
     # x2 = 3/4 x0 + 1/4 (x1 + dt L(x1))
     l1 = run(x1, m1, l2, None, 0.0, 0.25, 0.25 * dt)
-    # x1 = x0 + dt L(x0)
-    base = lam * (1 / 3) + l2 * 0.75
-    return run(x0, m0, l1, base, 1.0, 1.0, dt)
+    # x1 = x0 + dt L(x0); x0 also feeds u^{n+1} (1/3) and x2 (3/4)
+    return run(x0, m0, l1, lam, 1 / 3, 1.0, dt, base2=l2, a2=0.75)
+
+
+class _TauRows:
+    """Rows of a [TAU_ROWS][Q] buffer for the stages' per-q-cell d/d tau:
+    a full buffer is summed into the total (one reduction per TAU_ROWS
+    stages, in a fixed order) and reused."""
+    TAU_ROWS = 256
+
+    def __init__(self, Q, device):
+        self.buf = torch.empty((self.TAU_ROWS, Q), dtype=F64, device=device)
+        self.total = torch.zeros((), dtype=F64, device=device)
+        self.n = 0
+
+    def __call__(self):
+        if self.n == self.TAU_ROWS:
+            self.flush()
+        row = self.buf[self.n]
+        self.n += 1
+        return row
+
+    def flush(self):
+        if self.n:
+            self.total.add_(self.buf[:self.n].sum())
+        self.n = 0
+        return self.total
 
 
 class _Evolve(torch.autograd.Function):
@@ -146,8 +170,7 @@ class _Evolve(torch.autograd.Function):
         nws = int(s._L.boltx_adjoint_workspace_size(ctypes.byref(s._grid),
                                                     ctypes.byref(s._phys)))
         ws = torch.empty(max(nws, 8), dtype=torch.uint8, device=s.device)
-        tau_q = torch.empty(s._Q, dtype=F64, device=s.device)
-        tau_acc = torch.zeros((), dtype=F64, device=s.device)
+        tau_rows = _TauRows(s._Q, s.device)
         n, every, dt = ctx.n_steps, ctx.every, ctx.dt
         for seg in reversed(range(len(ctx.ckpts))):
             k0, k1 = seg * every, min(seg * every + every, n)
@@ -157,9 +180,9 @@ class _Evolve(torch.autograd.Function):
                 f, tape = _step_recorded(s, f, dt)
                 tapes.append(tape)
             for tape in reversed(tapes):
-                lam = _adjoint_step(s, tape, lam, dt, ws, tau_q, tau_acc)
+                lam = _adjoint_step(s, tape, lam, dt, ws, tau_rows)
         g_f0 = internal_to_interior(s, lam)
-        g_tau = tau_acc.to(ctx.tau_dtype).reshape(ctx.tau_shape)
+        g_tau = tau_rows.flush().to(ctx.tau_dtype).reshape(ctx.tau_shape)
         return g_f0, g_tau, None, None, None, None
 
 

@@ -6,7 +6,7 @@
 // C times the upwind face state (riemann.py:46-49; LLF has the same
 // derivative for a linear flux).  Given mu (the adjoint of the stage output
 // seen through L), adj_stage_kernel writes
-//   out = a*base + b*mu + c*J(u)^T mu,
+//   out = a*base + a2*base2 + b*mu + c*J(u)^T mu,
 // with J = dL/du, and adj_bgk_reduce_kernel the per-q-cell pieces of the
 // collision term's Jacobian and c * sum_p mu dL/dtau.
 //
@@ -197,8 +197,9 @@ struct AdjParams {
   const double* mom;  // moment block of u
   const double* lam;
   const double* base;
+  const double* base2;
   double* out;
-  double a, b, c;
+  double a, a2, b, c;
   double* gq;        // [5][Q] moment weights of the collision Jacobian
   double* tau_grad;  // [Q] or null
 };
@@ -410,6 +411,7 @@ __host__ __device__ inline void adj_stage_at(const AdjParams& ap, int i1, int i2
   }
   double x = ap.b * mu + ap.c * acc;
   if (ap.base) x += ap.a * ap.base[t];
+  if (ap.base2) x += ap.a2 * ap.base2[t];
   ap.out[t] = x;
 }
 
@@ -465,8 +467,10 @@ __host__ inline AdjParams adj_params(const boltx_grid* g, const boltx_physics* p
   ap.mom = adj->mom_u;
   ap.lam = (const double*)adj->lam;
   ap.base = (const double*)adj->base;
+  ap.base2 = (const double*)adj->base2;
   ap.out = (double*)adj->out;
   ap.a = adj->a;
+  ap.a2 = adj->a2;
   ap.b = adj->b;
   ap.c = adj->c;
   ap.gq = (double*)adj->workspace;

@@ -24,6 +24,7 @@
 #include "ql_launch.h"
 #include "qline_moments.h"
 #include "adjoint.h"
+#include "adjoint_lines.h"
 #include "bx_status.h"
 
 namespace bx {
@@ -1135,6 +1136,16 @@ extern "C" int boltx_adjoint_stage(const boltx_grid* grid, const boltx_physics* 
       hipLaunchKernelGGL(adj_bgk_reduce_kernel, dim3((unsigned)Q), dim3(256), 0, s, ap);
     }
     if ((rc = launch_status("adj_bgk_reduce_kernel"))) return rc;
+  }
+  if (ap.layout && phys->recon_q == RC_WENO5 && (ap.n1 == 64 || ap.n1 == 128)) {
+    // q1-line march: one wave per (p-cell, q2 segment); segments of at least
+    // 16 rows, enough of them for about 1024 workgroups
+    const long groups = (ap.P + 3) / 4;
+    const long nseg = std::max(1L, std::min((1024 + groups - 1) / groups, (long)(ap.n2 / 16)));
+    const dim3 grid((unsigned)(groups * nseg)), block(256);
+    if (ap.n1 == 64) hipLaunchKernelGGL(adj_line_kernel<1>, grid, block, 0, s, ap, (int)nseg);
+    else hipLaunchKernelGGL(adj_line_kernel<2>, grid, block, 0, s, ap, (int)nseg);
+    return launch_status("adj_line_kernel");
   }
   const dim3 nb((unsigned)(((long)ap.n1 * ap.P + 255) / 256), (unsigned)ap.n2);
   switch (phys->recon_q) {

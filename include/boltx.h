@@ -240,10 +240,11 @@ int boltx_check_finite(const boltx_grid *grid, const double *mom, int32_t *flag,
 /* Adjoint of one stage (FP64, fields off, periodic q: a grid with ghost
  * q-rows is rejected with BOLTX_EUNSUPPORTED).  For the stage
  * out = a*un + b*(w + cdt*L(u)) and an adjoint lam of its output, the caller
- * passes c = b*cdt and gets out = a*base + b*lam + c*J(u)^T lam, J = dL/du,
- * and, if tau_grad is non-NULL, tau_grad[q] = c * sum_p lam * dL/dtau for
- * each q-cell (the BGK source's dependence on its collision time).  base may
- * be NULL and may alias out; out must not alias u or lam.  The reference has
+ * passes c = b*cdt and gets out = a*base + a2*base2 + b*lam + c*J(u)^T lam,
+ * J = dL/du, and, if tau_grad is non-NULL, tau_grad[q] = c * sum_p lam *
+ * dL/dtau for each q-cell (the BGK source's dependence on its collision
+ * time).  base and base2 may be NULL and may alias out; out must not alias u
+ * or lam.  The reference has
  * no adjoint: this differentiates df_dt_fvm.py:33-97 with the BGK source
  * collision_operator.py:70-128 and the moments moments.py:5-22. */
 typedef struct boltx_adjoint {
@@ -255,6 +256,8 @@ typedef struct boltx_adjoint {
   double a, b, c;
   double *tau_grad;       /* [Q] or NULL */
   void *workspace;        /* boltx_adjoint_workspace_size() bytes */
+  const void *base2;      /* or NULL: a second term a2*base2 (SSP-RK3 reverses its */
+  double a2;              /* first stage into lam/3 + 3/4 lam_2 + ...) */
 } boltx_adjoint;
 
 int boltx_adjoint_stage(const boltx_grid *grid, const boltx_physics *phys,
