@@ -64,23 +64,28 @@ def _moments_of(s, f):
     return m
 
 
-def _step_recorded(s, f, dt):
+def _step_recorded(s, f, dt, need_out=True):
     """One step from f (not modified): returns (f_next, [(stage input, its
-    moment block), ...]) in stage order."""
+    moment block), ...]) in stage order.  With need_out False the last stage
+    (whose output the adjoint does not read) is skipped and f_next is None."""
     m0 = _moments_of(s, f)
     if s._stepper == 'RK2':
         uh = torch.empty_like(f)
         mh = torch.empty_like(m0)
         s._stage(f, f, None, uh, 0.0, 1.0, dt / 2, m0, mh)
-        out = torch.empty_like(f)
-        s._stage(uh, f, None, out, 0.0, 1.0, dt, mh, None)
+        out = None
+        if need_out:
+            out = torch.empty_like(f)
+            s._stage(uh, f, None, out, 0.0, 1.0, dt, mh, None)
         return out, [(f, m0), (uh, mh)]
     u1, m1 = torch.empty_like(f), torch.empty_like(m0)
     s._stage(f, f, None, u1, 0.0, 1.0, dt, m0, m1)
     u2, m2 = torch.empty_like(f), torch.empty_like(m0)
     s._stage(u1, u1, f, u2, 0.75, 0.25, dt, m1, m2)
-    out = torch.empty_like(f)
-    s._stage(u2, u2, f, out, 1 / 3, 2 / 3, dt, m2, None)
+    out = None
+    if need_out:
+        out = torch.empty_like(f)
+        s._stage(u2, u2, f, out, 1 / 3, 2 / 3, dt, m2, None)
     return out, [(f, m0), (u1, m1), (u2, m2)]
 
 
@@ -176,8 +181,9 @@ class _Evolve(torch.autograd.Function):
             k0, k1 = seg * every, min(seg * every + every, n)
             f = ctx.ckpts[seg]
             tapes = []
-            for _ in range(k0, k1):
-                f, tape = _step_recorded(s, f, dt)
+            for k in range(k0, k1):
+                # the segment's final state is the next checkpoint: not needed
+                f, tape = _step_recorded(s, f, dt, need_out=k + 1 < k1)
                 tapes.append(tape)
             for tape in reversed(tapes):
                 lam = _adjoint_step(s, tape, lam, dt, ws, tau_rows)
