@@ -58,6 +58,40 @@ def test_stage_adjoint_vs_oracle_autograd(gpu, layout, rq):
     assert err <= 1e-10 and terr <= 1e-10
 
 
+@pytest.mark.parametrize("n1,n2", [(128, 5), (64, 37)])
+def test_line_adjoint_shapes_and_second_base(gpu, n1, n2):
+    """The reverse-mode line kernel (q1-line layout, WENO5) on two q1 cells
+    per lane and on a q2 extent that splits into uneven segments, against
+    oracle autograd; the second base term adds a2 * base2."""
+    from bolt_amd import adjoint as A
+    c = maxwellian_case(N_q1=n1, N_q2=n2, N_p1=6, N_p2=4, p_dim=2, recon_q="weno5",
+                        recon_p="weno5", riemann="lax-friedrichs", tau=0.04,
+                        memory_layout="q1-lines", modes=((1, 1),), amps=(3e-2,), phases=(0.7,))
+    o = oracle_solver(c)
+    gen = torch.Generator().manual_seed(5)
+    u = o.interior().clone() * (1 + 0.2 * torch.rand(o.interior().shape, generator=gen, dtype=F64))
+    mu = torch.randn(u.shape, generator=gen, dtype=F64)
+    gu, gt = _oracle_vjp(o, u, mu, 0.04)
+    s = hip_solver(c)
+    U = A.interior_to_internal(s, u[:, 0].to(gpu))
+    MU = A.interior_to_internal(s, mu[:, 0].to(gpu))
+    mom = A._moments_of(s, U)
+    ws = torch.empty(_ws_bytes(s), dtype=torch.uint8, device=gpu)
+    tq = torch.empty(s._Q, dtype=F64, device=gpu)
+    out = A._adjoint_stage(s, U, mom, MU, None, 0.0, 0.0, 1.0, ws, tq)
+    got = A.internal_to_interior(s, out).cpu()
+    err = float((got - gu).abs().max() / gu.abs().max())
+    terr = abs(float(tq.sum()) - gt) / abs(gt)
+    print(f"n1={n1} n2={n2}: J^T mu rel Linf {err:.3e}, dL/dtau rel {terr:.3e}", flush=True)
+    assert err <= 1e-10 and terr <= 1e-10
+    B1 = torch.randn(U.shape, generator=gen, dtype=F64).to(gpu)
+    B2 = torch.randn(U.shape, generator=gen, dtype=F64).to(gpu)
+    one = A._adjoint_stage(s, U, mom, MU, B1, 0.3, 0.7, 0.01, ws, None)
+    two = A._adjoint_stage(s, U, mom, MU, B1, 0.3, 0.7, 0.01, ws, None, base2=B2, a2=-1.25)
+    d = float((two - one - (-1.25) * B2).abs().max() / two.abs().max())
+    assert d <= 1e-14, d
+
+
 @pytest.mark.parametrize("layout,n1,np1,np2", [("q1-lines", 40, 12, 10),
                                                ("reference", 10, 20, 15)])
 def test_stage_adjoint_ragged_reduction(gpu, layout, n1, np1, np2):
