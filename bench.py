@@ -412,16 +412,36 @@ def cpu_baseline(steps=3):
             o.strang_timestep(c.dt)
         return nq * nq * npp * npp * k / (time.perf_counter() - t0)
 
+    def rate_cases(cases, k):
+        cells, t = 0, 0.0
+        for c in cases:
+            d = c.domain
+            o = oracle_solver(c)
+            t0 = time.perf_counter()
+            for _ in range(k):
+                o.strang_timestep(c.dt)
+            t += time.perf_counter() - t0
+            cells += d.N_q1 * d.N_q2 * d.N_p1 * d.N_p2 * d.N_p3 * k
+        return cells / t
+
     v = rate(n, 128, 48, steps)
     v1 = rate(1, 32, 48, 1)
     torch.set_num_threads(n)
+    # the other workloads on bounded samples, same oracle, every thread
+    from tests.cases import case_2d3v, case_c5
+    v_em = rate_cases([case_2d3v(seed=0, N_q=12)], 1)
+    v_ens = rate_cases([c for c, _ in case_c5(seed=0, precision="fp64")[:8]], 1)
     return {"value": v, "unit": "phase-space cell-updates/s", "cores": n,
             "kind": "port",
             "sample": "FP64 oracle (torch CPU), 2D2V WENO5/LLF/BGK SSP-RK3 on the full "
                       "128^2 x 48^2 grid, %d steps, %d threads" % (steps, n),
             "cpu_model": _cpu_model(),
             "one_thread": {"value": v1, "sample": "32^2 x 48^2 cells, 1 step, 1 thread"},
-            "thread_speedup": v / v1}
+            "thread_speedup": v / v1,
+            "em_2d3v": {"value": v_em, "sample": "2D3V EM (PPM q, WENO5 p, FDTD) on 12^2 x 24^3, "
+                                                 "1 step, %d threads" % n},
+            "ensemble": {"value": v_ens, "sample": "the first 8 of the 96 1D2V members, 1 step "
+                                                   "each, one after another, %d threads" % n}}
 
 
 def probe(which):
