@@ -783,6 +783,44 @@ class nonlinear_solver(object):
             self.fields_solver.stage_fields = None
         self.time_elapsed += dt
 
+    def capture(self, n_steps, dt):
+        """Record ``n_steps`` steps of ``dt`` (the launches strang_timestep
+        issues) into a HIP graph (torch.cuda.CUDAGraph); each replay() then
+        advances f by those steps with one graph launch.  Periodic q, fields
+        off and the built-in terms only: the host-side work of the other
+        cases (field time levels, user callbacks, boundary functions of the
+        time) would not repeat on replay.  The launches use fixed buffers and
+        no memsets, so every replay repeats the eager steps exactly."""
+        if self._p_adv or self._general or self._G:
+            raise NotImplementedError('graph capture takes periodic q, fields off and the '
+                                      'built-in advection, source and collision time')
+        # the stages' work arrays exist before the capture (arrays allocated
+        # inside it would come from the graph's private pool)
+        names = ('u1', 'm1') if self._stepper == 'RK2' else ('u1', 'u2', 'm1', 'm2')
+        for _ in self._species_loop():
+            for nm in names:
+                if nm[0] == 'm':
+                    self._buf(nm, F64, (_lib.MOM_ROWS, self._Q))
+                else:
+                    self._buf(nm)
+        events, self._stage_events = self._stage_events, None
+        torch.cuda.synchronize()
+        t, d = self.time_elapsed, getattr(self, 'dt', None)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for _ in range(int(n_steps)):
+                self._advance(dt)
+        self.time_elapsed, self.dt, self._stage_events = t, d, events
+        self._graph, self._graph_steps, self._graph_dt = g, int(n_steps), float(dt)
+        return g
+
+    def replay(self):
+        """Run the captured steps (see capture), without a host sync."""
+        self._graph.replay()
+        for _ in range(self._graph_steps):
+            self.time_elapsed += self._graph_dt
+        self.dt = self._graph_dt
+
     def check_divergence(self):
         """timestep.py:28-38 (reads the device flag: one host sync)."""
         if int(self._flag.item()) != 0:
