@@ -25,15 +25,16 @@
 //    t-3..t-1; the indicator adjoint of cell t-3 is then complete and goes
 //    into rows t-4..t-2, and row t-4 is complete;
 //  * q1: the completed row's u and mu go through wave-private LDS lines
-//    (indicators, then faces, then indicator adjoints, then the gather), so
-//    no workgroup barrier is needed.
+//    (indicators, then faces, then the gather, which sums each indicator's
+//    adjoint from the three faces that name it), so no workgroup barrier is
+//    needed.
 #pragma once
 #include "adjoint.h"
 
 namespace bx {
 
 constexpr int AL_HALO = 4;   // periodic halo cells of the LDS lines
-constexpr int AL_LINES = 11; // U, M, R, IB, A1..A3, h1..h3, H
+constexpr int AL_LINES = 10; // U, M, R, IB, A1..A3, h1..h3
 
 // Substencil values of a right (or left) face window y[0..4].
 template <bool RIGHT>
@@ -106,7 +107,6 @@ __device__ __forceinline__ void al_q1_row(const double* ur, const double* mr, do
   double* IBl = Rl + LS;
   double* Al = IBl + LS;   // three lines
   double* hl = Al + 3 * LS;  // three lines
-  double* Hl = hl + 3 * LS;
   al_put<NR>(U, ur, lane, n1);
   al_put<NR>(M, mr, lane, n1);
   __builtin_amdgcn_wave_barrier();
@@ -151,17 +151,6 @@ __device__ __forceinline__ void al_q1_row(const double* ur, const double* mr, do
     al_put<NR>(hl + m * LS, x, lane, n1);
   }
   __builtin_amdgcn_wave_barrier();
-  // indicator adjoints: indicator m of face k is cell b(k) + 1 + m
-#pragma unroll
-  for (int k = 0; k < NR; ++k) {
-    const int c = lane + 64 * k;
-    // face k names cell c as indicator m when k = c + 3 - (m+1) (C > 0) or
-    // k = c + 2 - (m+1) (C <= 0)
-    const int k0 = P1 ? c + 2 : c + 1;
-    v[k] = (hl[k0] + hl[LS + k0 - 1]) + hl[2 * LS + k0 - 2];
-  }
-  al_put<NR>(Hl, v, lane, n1);
-  __builtin_amdgcn_wave_barrier();
   // gather: substencil adjoints of the five faces whose window holds the
   // cell, then the indicator adjoints of cells c-1, c, c+1
 #pragma unroll
@@ -175,9 +164,15 @@ __device__ __forceinline__ void al_q1_row(const double* ur, const double* mr, do
       for (int m = 0; m < 3; ++m)
         if (weno_sub_coef<P1>(m + 1, e) != 0.0) acc += Al[m * LS + f] * weno_sub_coef<P1>(m + 1, e);
     }
-    acc += Hl[c + 1] * beta_da(U[c], U[c + 1], U[c + 2]);
-    acc += Hl[c] * beta_db(U[c - 1], U[c], U[c + 1]);
-    acc += Hl[c - 1] * beta_dc(U[c - 2], U[c - 1], U[c]);
+    // indicator adjoint of cell x: face k names x as indicator m when
+    // k = x + 2 - m (C > 0) or k = x + 1 - m (C <= 0)
+    auto Hc = [&](int x) {
+      const int k0 = P1 ? x + 2 : x + 1;
+      return (hl[k0] + hl[LS + k0 - 1]) + hl[2 * LS + k0 - 2];
+    };
+    acc += Hc(c + 1) * beta_da(U[c], U[c + 1], U[c + 2]);
+    acc += Hc(c) * beta_db(U[c - 1], U[c], U[c + 1]);
+    acc += Hc(c - 1) * beta_dc(U[c - 2], U[c - 1], U[c]);
     out[k] = acc;
   }
   __builtin_amdgcn_wave_barrier();
