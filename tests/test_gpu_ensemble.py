@@ -75,6 +75,33 @@ def test_ensemble_matches_single_solver_runs(gpu):
     assert worst <= 1e-10
 
 
+@pytest.mark.parametrize("recon", ["minmod", "ppm", "piecewise-constant"])
+def test_ensemble_other_reconstructions_match_single_solver(gpu, recon):
+    """The chunk kernel's minmod, PPM and piecewise-constant forms (odd line
+    lengths take the one-cell path) against the single-problem solver."""
+    from bolt_amd.ensemble import Ensemble
+    rng = np.random.default_rng(4)
+    cases = []
+    for n1, npp in [(32, 8), (37, 6), (128, 6), (16, 8)]:
+        c = maxwellian_case(N_q1=n1, N_q2=1, N_p1=npp, N_p2=npp, p_dim=2, p_max=10.0,
+                            modes=((1, 0),), amps=(float(rng.uniform(1e-3, 1e-2)),),
+                            phases=(float(rng.uniform(0, 6.28)),), recon_q=recon,
+                            recon_p=recon, riemann="upwind-flux", stepper="SSP-RK3",
+                            tau=0.05, precision="fp64")
+        c.dt = 0.4 / n1 / 10.0
+        cases.append(c)
+    solo = [hip_solver(c) for c in cases]
+    ens = Ensemble([hip_solver(c) for c in cases], [c.dt for c in cases], autotune=False)
+    ens.step(5)
+    for s, c in zip(solo, cases):
+        for _ in range(5):
+            s.strang_timestep(c.dt)
+    worst = max(_rel_linf(ens.state(i), ens._lines(s)) for i, s in enumerate(solo))
+    print(f"{recon}: ensemble vs single-problem solver after 5 steps: rel Linf {worst:.3e}",
+          flush=True)
+    assert worst <= 1e-10
+
+
 def test_ensemble_fp32_bulk_vs_fp64(gpu):
     from bolt_amd.ensemble import Ensemble
     c64 = _members("fp64", SIZES, seed=3)
