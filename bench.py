@@ -316,9 +316,10 @@ def fp32_bulk_error(case_fn, n_steps):
 def run_ensemble(steps=200, sample_steps=10):
     """The ensemble workload: 96 1D2V members of uneven sizes (N_q1 in
     {32..256}, N_p in {24..64}^2, tau in [1e-3, 1], own dt each), WENO5 + LLF
-    + BGK, SSP-RK3, FP32-bulk with FP64 moments, 200 steps in persistent
-    ensemble launches with the autotuned work-list shape, replayed as one HIP
-    graph (the eager time is reported too); beside it the same
+    + BGK, SSP-RK3, FP32-bulk with FP64 moments, 200 steps of the ensemble
+    launches (a chunk-stage and a finalize launch per stage for all members)
+    at the autotuned work shape, replayed as one HIP graph (the eager time is
+    reported too); beside it the same
     members stepped one after another by the single-problem solver in plain
     FP64 (timed over ``sample_steps`` steps each)."""
     import torch
