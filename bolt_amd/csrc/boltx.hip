@@ -571,19 +571,21 @@ static int launch_qc(const QCParams& qp, int rq, int rp, bool padv, hipStream_t 
 constexpr int ML_CHUNKS = 8;
 
 // Number of q2 segments for ``groups`` workgroups per segment when ``slots``
-// workgroups are resident at once: segments of about 16 rows (measured at
-// 128^2 q x 48^2 p on an MI355X, 4 to 12 segments lie within 4 % of each
-// other, 8 is best in FP32 and within 1 % of best in FP64), more of them
-// (down to 4 rows) while the launch would fill fewer than two waves of
-// workgroups.  BOLTX_QL_SEGMENTS overrides the count (tuning sweeps).
+// workgroups are resident at once: segments of about 32 rows (a segment
+// re-reads a 4-row window halo, so longer ones read less), split in two
+// while the launch would not fill the resident slots once, down to 4 rows.
+// Measured at 128^2 q x 48^2 p on an MI355X (one-step graph replays): 4
+// segments 0.740 ms per FP64 step, 8 segments 0.752, 2 segments 0.782, 6
+// (uneven) 0.764; FP32-bulk 0.383-0.386 for 4 and 8.  BOLTX_QL_SEGMENTS
+// overrides the count (tuning sweeps).
 static int ql_segments(int n2, long groups, long slots) {
   if (const char* e = getenv("BOLTX_QL_SEGMENTS")) {
     const int v = atoi(e);
     if (v >= 1) return v < n2 ? v : n2;
   }
   if (n2 <= 1) return 1;
-  int ns = n2 / 16 > 1 ? n2 / 16 : 1;
-  while (groups * ns < 2 * slots && n2 / (ns + 1) >= 4) ++ns;
+  int ns = n2 / 32 > 1 ? n2 / 32 : 1;
+  while (groups * ns < slots && n2 / (2 * ns) >= 4) ns *= 2;
   return ns;
 }
 
