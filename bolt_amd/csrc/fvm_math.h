@@ -68,11 +68,45 @@ __host__ __device__ inline T mc_slope(T ym, T y, T yp) {
   return T(0.25) * aabs(sx + sy) * (sx + sz) * mn;
 }
 
-// ppm.py:57-98 on the five-cell stencil y0..y4 of the centre cell y2.
+// ppm.py:58-80: the limited slope of the centre cell y of ym, y, yp.
 template <typename T>
-__host__ __device__ inline void ppm_faces(T y0, T y1, T y2, T y3, T y4, T& left, T& right) {
+__host__ __device__ inline T ppm_dq(T ym, T y, T yp) {
   // no contraction: a*b - c*d forms would otherwise round differently
   // wherever the compiler fuses the other product
+#pragma clang fp contract(off)
+  const T dm = T(2) * (y - ym), dp = T(2) * (yp - y);
+  const T D = T(0.5) * (yp - ym);
+  return (dp * dm <= T(0)) ? T(0)
+         : (D > T(0) ? T(1) : T(-1)) * amin(aabs(D), amin(aabs(dm), aabs(dp)));
+}
+
+// ppm.py:82-98: the two faces of the centre cell y2 of y1, y2, y3 from the
+// limited slopes DQ1, DQ2, DQ3 of the three cells.
+template <typename T>
+__host__ __device__ inline void ppm_faces_dq(T y1, T y2, T y3, T DQ1, T DQ2, T DQ3, T& left,
+                                             T& right) {
+#pragma clang fp contract(off)
+  const T sixth = T(1.0 / 6.0);
+  T l = T(0.5) * (y2 + y1) - (DQ2 - DQ1) * sixth;
+  T r = T(0.5) * (y3 + y2) - (DQ3 - DQ2) * sixth;
+  const bool corr1 = (r - y2) * (y2 - l) <= T(0);
+  const T qd = r - l;
+  const T qe = T(6) * (y2 - T(0.5) * (r + l));
+  const bool corr2 = qd * (qd - qe) < T(0);
+  const bool corr3 = qd * (qd + qe) < T(0);
+  if (corr1) { l = y2; r = y2; }
+  if (corr2) l = T(3) * y2 - T(2) * r;
+  else if (corr3) r = T(3) * y2 - T(2) * l;
+  left = l;
+  right = r;
+}
+
+// ppm.py:57-98 on the five-cell stencil y0..y4 of the centre cell y2: the
+// arithmetic of ppm_faces_dq on ppm_dq of y1, y2, y3, written out in one
+// piece, which the FP64 stage kernels schedule better (the two forms agree
+// bit for bit).
+template <typename T>
+__host__ __device__ inline void ppm_faces(T y0, T y1, T y2, T y3, T y4, T& left, T& right) {
 #pragma clang fp contract(off)
   const T d0 = T(2) * (y1 - y0), d1 = T(2) * (y2 - y1);
   const T d2 = T(2) * (y3 - y2), d3 = T(2) * (y4 - y3);

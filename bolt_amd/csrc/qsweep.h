@@ -19,9 +19,10 @@
 // The reference's left-face formula on a window z0..z4 equals its right-face
 // formula on the reversed window z4..z0 with the three indicators reversed
 // (weno5.py:73-96; the indicators themselves keep their forward formula,
-// weno5.py:58-71), so both cases are one formula on selected operands and
-// lines of either sign share a wave without divergence.  LLF equals upwind
-// for this linear flux (C is the same on both sides of every face).
+// weno5.py:58-71), so a line with C <= 0 is marched backwards with the
+// right-face formula, and lines of either sign share a wave without
+// divergence.  LLF equals upwind for this linear flux (C is the same on both
+// sides of every face).
 //
 // LDS: the q-cell's block with each p1-line padded to n_p1 + 1 floats (an odd
 // pitch for n_p1 = 24, so a wave's 64 lines hit 64 distinct banks), the
@@ -64,18 +65,6 @@ __device__ __forceinline__ float qs_face(float y0, float y1, float y2, float y3,
   return num * __builtin_amdgcn_rcpf(den);
 }
 
-// Upwind state of face j+1/2 from the window y[k] = f[j-3+k], k = 0..6, and
-// the indicator quantities kp[k] of cells j-1..j+2: the right face of cell j
-// (y[1..5], kp[0..2]) when pos, else the left face of cell j+1 as the
-// right-face formula on the reversed window (y[6..2], kp[3..1]); operands are
-// selected per lane, then one face is evaluated.
-__device__ __forceinline__ float qs_select_state(const float y[7], const float kp[4], bool pos) {
-  const float a0 = pos ? y[1] : y[6], a1 = pos ? y[2] : y[5], a2 = pos ? y[3] : y[4];
-  const float a3 = pos ? y[4] : y[3], a4 = pos ? y[5] : y[2];
-  const float k1 = pos ? kp[0] : kp[3], k2 = pos ? kp[1] : kp[2], k3 = pos ? kp[2] : kp[1];
-  return qs_face(a0, a1, a2, a3, a4, k1, k2, k3);
-}
-
 // Flux difference C*(s(i+1/2) - s(i-1/2)) along a q axis for the cell at the
 // centre of a[k] = f[i-3+k], with the upwind side chosen per lane: both faces
 // are right-face reconstructions of selected windows, the lower face of
@@ -83,19 +72,22 @@ __device__ __forceinline__ float qs_select_state(const float y[7], const float k
 // (C <= 0: left face of cell i), the upper face of a[1..5] or of the reversed
 // a[6..2].  PPM, minmod and piecewise-constant faces are mirror-symmetric
 // (ppm.py:83-98: of the two corrections at most one applies); for WENO5 the
-// forward indicators are assigned in reverse order (weno5.py:58-96).  The
-// lower face state is returned in ``lower`` (f_left / f_bot of
-// df_dt_fvm.py:79-80).
+// forward indicators are assigned in reverse order (weno5.py:58-96).  For
+// the other reconstructions both windows are cut from one upwind-ordered
+// window W[k] = a[k] (C > 0) or a[6-k]: the faces of W0..W4 and W1..W5 are
+// the lower and upper faces (C > 0) or the upper and lower ones, and PPM's
+// limited slopes of W1..W4 serve both.  The lower face state is returned in
+// ``lower`` (f_left / f_bot of df_dt_fvm.py:79-80).
 template <int RQ>
 __device__ __forceinline__ float qs_q_flux_diff(const float a[7], float C, bool pos, float& lower) {
-  float L[5], U[5];
-#pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    L[k] = pos ? a[k] : a[5 - k];
-    U[k] = pos ? a[k + 1] : a[6 - k];
-  }
   float sL, sU;
   if constexpr (RQ == RC_WENO5) {
+    float L[5], U[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      L[k] = pos ? a[k] : a[5 - k];
+      U[k] = pos ? a[k + 1] : a[6 - k];
+    }
     float b[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) b[k] = weno_beta<float>(a[k], a[k + 1], a[k + 2]);
@@ -104,8 +96,22 @@ __device__ __forceinline__ float qs_q_flux_diff(const float a[7], float C, bool 
     sU = weno_combine(U[0], U[1], U[2], U[3], U[4], pos ? b[1] : b[4], pos ? b[2] : b[3],
                       pos ? b[3] : b[2], true);
   } else {
-    sL = face_value<RQ>(L[0], L[1], L[2], L[3], L[4], true);
-    sU = face_value<RQ>(U[0], U[1], U[2], U[3], U[4], true);
+    float W[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) W[k] = pos ? a[k] : a[6 - k];
+    float F0, F1;
+    if constexpr (RQ == RC_PPM) {
+      float dq[4], l;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) dq[k] = ppm_dq(W[k], W[k + 1], W[k + 2]);
+      ppm_faces_dq(W[1], W[2], W[3], dq[0], dq[1], dq[2], l, F0);
+      ppm_faces_dq(W[2], W[3], W[4], dq[1], dq[2], dq[3], l, F1);
+    } else {
+      F0 = face_value<RQ>(W[0], W[1], W[2], W[3], W[4], true);
+      F1 = face_value<RQ>(W[1], W[2], W[3], W[4], W[5], true);
+    }
+    sL = pos ? F0 : F1;
+    sU = pos ? F1 : F0;
   }
   lower = sL;
   return C * sU - C * sL;
@@ -151,52 +157,56 @@ __device__ inline void qs_sweep(const StageParams& sp, const QCell<float>& c, co
     }
     const float Cf = float(C);
     const bool pos = Cf > 0.0f;
+    // The line is marched in its upwind direction: march cell m is line cell
+    // m (C > 0) or n-1-m, so the upwind state of the face above march cell m
+    // is always the right-face formula on march cells m-2 .. m+2 with the
+    // indicators of m-1, m, m+1 (for C <= 0 that is the left face of line
+    // cell n-1-m on its reversed window).
     const int j0 = seg * QS_SEG;
     const int len = n - j0 < QS_SEG ? n - j0 : QS_SEG;
-    // window y[k] = cell j0-4+k, k = 0 .. QS_SEG+6, periodic along the line
-    // (a short last segment loads cells it does not use); the line offset
-    // advances by stride and wraps to 0 at the line's end
+    const int dstr = pos ? stride : -stride;
+    // window y[k] = march cell j0-4+k, k = 0 .. QS_SEG+6, periodic along the
+    // line (a short last segment loads cells it does not use); the line offset
+    // moves by one stride per cell and wraps at the line's ends
     float y[QS_SEG + 7];
     {
-      int j = j0 - 4;
-      j = n >= 4 ? (j < 0 ? j + n : j) : ((j % n) + n) % n;
+      int j = pos ? j0 - 4 : n + 3 - j0;
+      j = n >= 4 ? (j < 0 ? j + n : (j >= n ? j - n : j)) : ((j % n) + n) % n;
       int off = j * stride;
       const int end = n * stride;
+      const int wrap_at = pos ? end : -stride, wrap_to = pos ? 0 : end - stride;
 #pragma unroll
       for (int k = 0; k < QS_SEG + 7; ++k) {
         y[k] = blk[base + off];
-        off += stride;
-        off = off == end ? 0 : off;
+        off += dstr;
+        off = off == wrap_at ? wrap_to : off;
       }
     }
-    // indicator quantity kp[k] of cell j0-3+k (window y[k..k+2]), k = 0 .. len+4
+    // indicator quantity kp[k] of march cell j0-3+k, k = 0 .. len+4: the
+    // forward formula on the three cells in line order
     float kp[QS_SEG + 5];
 #pragma unroll
-    for (int k = 0; k < QS_SEG + 5; ++k) kp[k] = qs_kappa(y[k], y[k + 1], y[k + 2]);
-    // flux through the lower face of cell j0 (face j+1/2 of j = j0-1: window
-    // cells j0-4 .. j0+2, indicators of cells j0-2 .. j0+1)
-    float wl[7], kl[4];
-#pragma unroll
-    for (int k = 0; k < 7; ++k) wl[k] = y[k];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) kl[k] = kp[k + 1];
-    float Flo = Cf * qs_select_state(wl, kl, pos);
+    for (int k = 0; k < QS_SEG + 5; ++k) {
+      const float lo = pos ? y[k] : y[k + 2], hi = pos ? y[k + 2] : y[k];
+      kp[k] = qs_kappa(lo, y[k + 1], hi);
+    }
+    // flux through the face below march cell j0 (window y[1..5], indicators
+    // kp[1..3])
+    float Flo = Cf * qs_face(y[1], y[2], y[3], y[4], y[5], kp[1], kp[2], kp[3]);
+    int at = base + (pos ? j0 : n - 1 - j0) * stride;
 #pragma unroll
     for (int s = 0; s < QS_SEG; ++s) {
       if (s < len) {
-        // face (j0+s)+1/2: window cells j0+s-3 .. j0+s+3 = y[s+1 .. s+7],
-        // indicators of cells j0+s-1 .. j0+s+2 = kp[s+2 .. s+5]
-        float w[7], kk[4];
-#pragma unroll
-        for (int k = 0; k < 7; ++k) w[k] = y[s + 1 + k];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) kk[k] = kp[s + 2 + k];
-        const float Fhi = Cf * qs_select_state(w, kk, pos);
-        const int at = base + (j0 + s) * stride;
-        const float dflux = (Fhi - Flo) * inv_dp;
+        // face above march cell j0+s: window y[s+2 .. s+6], indicators
+        // kp[s+2 .. s+4]; the line cell's flux difference is the march
+        // difference, negated when the march runs backwards
+        const float Fhi = Cf * qs_face(y[s + 2], y[s + 3], y[s + 4], y[s + 5], y[s + 6],
+                                       kp[s + 2], kp[s + 3], kp[s + 4]);
+        const float dflux = (pos ? Fhi - Flo : Flo - Fhi) * inv_dp;
         if (AX == 0) acc[at] = dflux;
         else acc[at] = acc[at] + dflux;
         Flo = Fhi;
+        at += dstr;
       }
     }
   }

@@ -239,3 +239,25 @@ extern "C" int host_ql_faces(int recon, int precision, int right, const double* 
   }
   return 0;
 }
+
+// ppm_faces against the split form the FP32 p-sweep kernel uses for its
+// q-faces (ppm_dq of each cell, then ppm_faces_dq) on n five-cell windows
+// y[5*i .. 5*i+4]: the number of windows whose faces differ in any bit.
+template <typename T>
+static int ppm_split_mismatch(const double* y, int n) {
+  int bad = 0;
+  for (int i = 0; i < n; ++i) {
+    T a[5];
+    for (int k = 0; k < 5; ++k) a[k] = T(y[5 * i + k]);
+    T l0, r0, l1, r1;
+    ppm_faces(a[0], a[1], a[2], a[3], a[4], l0, r0);
+    ppm_faces_dq(a[1], a[2], a[3], ppm_dq(a[0], a[1], a[2]), ppm_dq(a[1], a[2], a[3]),
+                 ppm_dq(a[2], a[3], a[4]), l1, r1);
+    bad += (std::memcmp(&l0, &l1, sizeof(T)) != 0 || std::memcmp(&r0, &r1, sizeof(T)) != 0);
+  }
+  return bad;
+}
+
+extern "C" int host_ppm_split_mismatch(int precision, const double* y, int n) {
+  return precision == 0 ? ppm_split_mismatch<double>(y, n) : ppm_split_mismatch<float>(y, n);
+}

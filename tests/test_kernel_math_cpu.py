@@ -178,3 +178,29 @@ def test_qline_faces_match_reference_faces(rq, precision):
         lib.host_ql_faces(H.L.RECON[rq], precision, right, y.ctypes.data, out.ctypes.data, n)
         err = np.abs(out - ref).max() / np.abs(ref).max()
         assert err <= (1e-13 if precision == 0 else 1e-5), (rq, right, err)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_ppm_split_form_is_bitwise_ppm_faces(precision):
+    """The FP32 p-sweep kernel forms its two q-faces per cell from four shared
+    limited slopes (ppm_dq, then ppm_faces_dq); both faces must equal
+    ppm_faces bit for bit.  The windows mix smooth data with values on a
+    coarse grid, so zero slopes, flat runs, extrema and each of the three
+    corrections (ppm.py:89-98) occur."""
+    import ctypes
+    from tests.hostcheck import harness as H
+    lib = H.lib()
+    lib.host_ppm_split_mismatch.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
+    lib.host_ppm_split_mismatch.restype = ctypes.c_int
+    rng = np.random.default_rng(17)
+    n = 20000
+    smooth = 1.0 + 0.3 * rng.standard_normal((n, 5))
+    coarse = np.round(rng.standard_normal((n, 5)) * 2) / 2
+    y = np.ascontiguousarray(np.concatenate([smooth, coarse]))
+    # the grid data must exercise the corrections: count windows where the
+    # oracle's corrected faces differ from the unlimited parabola's
+    T5 = [torch.from_numpy(y[:, k].copy()) for k in range(5)]
+    L, R = bo.ppm_faces(*T5)
+    plain_l = 0.5 * (T5[2] + T5[1])
+    assert int((L != plain_l).sum()) > n // 10
+    assert lib.host_ppm_split_mismatch(precision, y.ctypes.data, len(y)) == 0
