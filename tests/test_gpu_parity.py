@@ -221,6 +221,23 @@ def test_2d3v_fields_fp32_bulk_large_p_block(gpu):
     assert l2 <= 1e-5
 
 
+def test_2d3v_fields_fp32_bulk_short_uneven_p_lines(gpu):
+    """The FP32-bulk p-line sweeps on lines shorter than the WENO5 stencil
+    (3 and 2 cells, wrapping several times within one window) and of 5 cells
+    (one short segment), marched forwards and backwards (v x B gives both
+    signs of C along p1 and p2): against the FP64 oracle."""
+    c64 = case_2d3v(seed=9, N_q=8, N_p1=3, N_p2=2, N_p3=5)
+    c32 = case_2d3v(seed=9, N_q=8, N_p1=3, N_p2=2, N_p3=5, precision="fp32-bulk")
+    o, nls = oracle_solver(c64), hip_solver(c32)
+    for _ in range(4):
+        o.strang_timestep(c64.dt)
+        nls.strang_timestep(c32.dt)
+    linf, l2 = _cmp(nls, o)
+    print(f"2D3V FP32-bulk, 3 x 2 x 5 p-cells, 4 steps vs FP64 oracle: rel L2 {l2:.3e} "
+          f"rel Linf {linf:.3e}", flush=True)
+    assert l2 <= 1e-5
+
+
 def test_2d3v_fields_large_p_block_fp64(gpu):
     """An FP64 p-block too large for two workgroups per CU (22^3 p-cells,
     83 KB) runs as one 16-wave workgroup per q-cell: per-step parity."""
