@@ -282,11 +282,16 @@ __global__ __launch_bounds__(NT) void qsweep_kernel(QCParams qp) {
   double mac[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   double jac[3] = {0.0, 0.0, 0.0};
   const float rcp1 = 1.0f / float(sp.n_p1), rcp2 = 1.0f / float(sp.n_p2);
+  // p = threadIdx.x + k NT: its indices (j1, j2, j3) advance by the uniform
+  // steps of NT = q1 n1 + d1, q1 = q2 n2 + d2, with one carry each
+  const int n1 = sp.n_p1, n2 = sp.n_p2;
+  const int q1 = NT / n1, d1 = NT - q1 * n1;
+  const int q2 = q1 / n2, d2 = q1 - q2 * n2;
+  int j1, j2, j3;
+  j3 = qs_divmod(qs_divmod((int)threadIdx.x, n1, rcp1, j1), n2, rcp2, j2);
   // natural p order: a wave's loads and stores are contiguous runs of 64
   for (int p = threadIdx.x; p < sp.P; p += NT) {
-    int j1, j2;
-    const int j23 = qs_divmod(p, sp.n_p1, rcp1, j1);
-    const int j3 = qs_divmod(j23, sp.n_p2, rcp2, j2);
+    const int j23 = j2 + n2 * j3;
     const double v1 = centre(sp.p1s, j1, sp.dp1);
     const double v2 = centre(sp.p2s, j2, sp.dp2);
     const double v3 = centre(sp.p3s, j3, sp.dp3);
@@ -334,6 +339,13 @@ __global__ __launch_bounds__(NT) void qsweep_kernel(QCParams qp) {
     mac[2] += xd * v2;
     mac[3] += xd * v3;
     mac[4] += (0.5 * xd) * ((v1 * v1 + v2 * v2) + v3 * v3);
+    j1 += d1;
+    const int c1 = j1 >= n1 ? 1 : 0;
+    j1 -= c1 ? n1 : 0;
+    j2 += d2 + c1;
+    const int c2 = j2 >= n2 ? 1 : 0;
+    j2 -= c2 ? n2 : 0;
+    j3 += q2 + c2;
   }
   qc_finish<NW>(sp, q, mac, jac, red, wave, lane);
 }
